@@ -1,0 +1,483 @@
+// sht_capi.hip -- C ABI of libexosht.so (declared in include/exo_sht.h).
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/exo_sht.h"
+#include "sht_internal.h"
+
+using namespace exo;
+
+struct exo_sht_plan_s {
+  Tables tb;
+  void* scratch = nullptr;  // partial direct sums of multi-block truncations
+  size_t scratch_bytes = 0;
+  void* stage = nullptr;    // device staging of the host drop-in calls
+  size_t stage_bytes = 0;
+  int nlev = 0;
+  std::mutex mu;
+};
+
+#define HIPCK(x)                        \
+  do {                                  \
+    if ((x) != hipSuccess) return EXO_ERR_HIP; \
+  } while (0)
+
+static int nlat_of(int ntru) {
+  switch (ntru) {
+    case 21: return 32;
+    case 42: return 64;
+    case 63: return 96;
+    case 127: return 192;
+    default: return -1;
+  }
+}
+
+extern "C" const char* exo_strerror(int code) {
+  switch (code) {
+    case EXO_OK: return "ok";
+    case EXO_ERR_ARG: return "invalid argument";
+    case EXO_ERR_UNSUPPORTED: return "unsupported truncation or length";
+    case EXO_ERR_LAUNCH: return "kernel launch failed";
+    case EXO_ERR_SCRATCH: return "scratch buffer too small";
+    case EXO_ERR_HIP: return "HIP runtime error";
+    case EXO_ERR_NOINIT: return "exo_legini has not been called";
+    default: return "unknown error";
+  }
+}
+
+extern "C" int exo_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+static void free_tables(Tables& t) {
+  void* ps[] = {t.sid, t.gwd, t.tab, t.filt64, t.filt32, t.skgpsp, t.skspgp, t.tw64, t.tw32};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  for (int i = 0; i < NTAB; ++i) {
+    if (t.inv_bf[i]) (void)hipFree(t.inv_bf[i]);
+    if (t.inv_f32[i]) (void)hipFree(t.inv_f32[i]);
+    if (t.fwd_bf[i]) (void)hipFree(t.fwd_bf[i]);
+    if (t.fwd_f32[i]) (void)hipFree(t.fwd_f32[i]);
+  }
+  t = Tables();
+}
+
+static int build_tables(Tables& t, int ntru, const double* sid, const double* gwd,
+                        const double* skgpsp, const double* skspgp, double plavor) {
+  const int nlat = nlat_of(ntru);
+  if (nlat < 0) return EXO_ERR_UNSUPPORTED;
+  t.ntru = ntru;
+  t.nlat = nlat;
+  t.nlon = 2 * nlat;
+  t.npair = nlat / 2;
+  t.ncsp = (ntru + 1) * (ntru + 2) / 2;
+  t.kbmax = (ntru + 1 + 31) / 32;
+  t.rbmax = ((ntru + 2) / 2 + 15) / 16;
+  t.kpad = ((t.npair + 31) / 32) * 32;
+  t.plavor = plavor;
+  const long tsz = (long)t.npair * t.ncsp;
+  HIPCK(hipMalloc(&t.sid, sizeof(double) * nlat));
+  HIPCK(hipMalloc(&t.gwd, sizeof(double) * nlat));
+  HIPCK(hipMalloc(&t.tab, sizeof(double) * NTAB * tsz));
+  HIPCK(hipMalloc(&t.filt64, sizeof(double) * NTAB * tsz));
+  HIPCK(hipMalloc(&t.filt32, sizeof(float) * NTAB * tsz));
+  HIPCK(hipMalloc(&t.skgpsp, sizeof(double) * (ntru + 1)));
+  HIPCK(hipMalloc(&t.skspgp, sizeof(double) * (ntru + 1)));
+  HIPCK(hipMalloc(&t.tw64, sizeof(double) * 2 * t.nlon));
+  HIPCK(hipMalloc(&t.tw32, sizeof(float) * 2 * t.nlon));
+  if (sid && gwd) {
+    HIPCK(hipMemcpy(t.sid, sid, sizeof(double) * nlat, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(t.gwd, gwd, sizeof(double) * nlat, hipMemcpyHostToDevice));
+  } else {
+    k_inigau<<<(nlat / 2 + 63) / 64, 64>>>(nlat, t.sid, t.gwd);
+    HIPCK(hipGetLastError());
+  }
+  std::vector<double> ones(ntru + 1, 1.0);
+  HIPCK(hipMemcpy(t.skgpsp, skgpsp ? skgpsp : ones.data(), sizeof(double) * (ntru + 1),
+                  hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(t.skspgp, skspgp ? skspgp : ones.data(), sizeof(double) * (ntru + 1),
+                  hipMemcpyHostToDevice));
+  dim3 lg((t.npair + 63) / 64, ntru + 1);
+  k_legini<<<lg, 64>>>(ntru, t.npair, t.sid, t.gwd, t.tab);
+  HIPCK(hipGetLastError());
+  const long nf = NTAB * tsz;
+  k_filtered<<<(unsigned)((nf + 255) / 256), 256>>>(ntru, t.npair, t.tab, t.skgpsp, t.skspgp,
+                                                    t.filt64, t.filt32);
+  HIPCK(hipGetLastError());
+  t.inv_elems = (long)(t.npair / 8) * (ntru + 1) * t.kbmax * 512;
+  t.fwd_elems = (long)(ntru + 1) * 2 * t.rbmax * 16 * t.kpad;
+  const int inv_tabs[] = {TQI, TQJ, TQU, TQV};
+  for (int tt : inv_tabs) {
+    HIPCK(hipMalloc(&t.inv_bf[tt], sizeof(uint16_t) * 3 * t.inv_elems));
+    HIPCK(hipMalloc(&t.inv_f32[tt], sizeof(float) * t.inv_elems));
+    k_pack_inv<<<(unsigned)((t.inv_elems + 255) / 256), 256>>>(ntru, t.npair, t.kbmax, tt,
+                                                              t.filt64, t.inv_bf[tt],
+                                                              t.inv_f32[tt]);
+    HIPCK(hipGetLastError());
+  }
+  const int fwd_tabs[] = {TQC, TQE, TQM, TQQ};
+  for (int tt : fwd_tabs) {
+    HIPCK(hipMalloc(&t.fwd_bf[tt], sizeof(uint16_t) * 3 * t.fwd_elems));
+    HIPCK(hipMalloc(&t.fwd_f32[tt], sizeof(float) * t.fwd_elems));
+    k_pack_fwd<<<(unsigned)((t.fwd_elems + 255) / 256), 256>>>(
+        ntru, t.npair, t.rbmax, t.kpad, tt, t.filt64, t.fwd_bf[tt], t.fwd_f32[tt]);
+    HIPCK(hipGetLastError());
+  }
+  k_twiddle<<<(t.nlon + 255) / 256, 256>>>(t.nlon, t.tw64, t.tw32);
+  HIPCK(hipGetLastError());
+  HIPCK(hipDeviceSynchronize());
+  return EXO_OK;
+}
+
+// full [8][nlat][ncsp] host copy of the tables, southern half by symmetry
+static int copy_tables(const Tables& t, double* out) {
+  const long tsz = (long)t.npair * t.ncsp;
+  std::vector<double> h(NTAB * tsz);
+  HIPCK(hipMemcpy(h.data(), t.tab, sizeof(double) * NTAB * tsz, hipMemcpyDeviceToHost));
+  for (int tt = 0; tt < NTAB; ++tt)
+    for (int lat = 0; lat < t.nlat; ++lat) {
+      const bool south = lat >= t.npair;
+      const int pl = south ? t.nlat - 1 - lat : lat;
+      for (int m = 0, w = 0; m <= t.ntru; ++m)
+        for (int n = m; n <= t.ntru; ++n, ++w) {
+          double v = h[tt * tsz + (long)pl * t.ncsp + w];
+          if (south) {
+            const bool odd = ((n - m) & 1) != 0;
+            if (odd != table_is_q(tt)) v = -v;
+          }
+          out[((long)tt * t.nlat + lat) * t.ncsp + w] = v;
+        }
+    }
+  return EXO_OK;
+}
+
+static int grow(void** p, size_t* have, size_t need) {
+  if (*have >= need) return EXO_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *have = 0;
+  HIPCK(hipMalloc(p, need));
+  *have = need;
+  return EXO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plans
+// ---------------------------------------------------------------------------
+extern "C" int exo_sht_plan_create(exo_sht_plan_t* plan, int ntru, const double* sid,
+                                   const double* gwd, const double* skgpsp,
+                                   const double* skspgp, double plavor) {
+  if (!plan) return EXO_ERR_ARG;
+  *plan = nullptr;
+  exo_sht_plan_s* p = new exo_sht_plan_s();
+  int rc = build_tables(p->tb, ntru, sid, gwd, skgpsp, skspgp, plavor);
+  if (rc != EXO_OK) {
+    free_tables(p->tb);
+    delete p;
+    return rc;
+  }
+  *plan = p;
+  return EXO_OK;
+}
+
+extern "C" int exo_sht_plan_destroy(exo_sht_plan_t p) {
+  if (!p) return EXO_ERR_ARG;
+  (void)hipDeviceSynchronize();
+  free_tables(p->tb);
+  if (p->scratch) (void)hipFree(p->scratch);
+  if (p->stage) (void)hipFree(p->stage);
+  delete p;
+  return EXO_OK;
+}
+
+extern "C" int exo_sht_plan_dims(exo_sht_plan_t p, int* dims) {
+  if (!p || !dims) return EXO_ERR_ARG;
+  dims[0] = p->tb.ntru;
+  dims[1] = p->tb.nlat;
+  dims[2] = p->tb.nlon;
+  dims[3] = p->tb.ncsp;
+  return EXO_OK;
+}
+
+extern "C" int exo_sht_plan_grid(exo_sht_plan_t p, double* sid, double* gwd) {
+  if (!p) return EXO_ERR_ARG;
+  if (sid) HIPCK(hipMemcpy(sid, p->tb.sid, sizeof(double) * p->tb.nlat, hipMemcpyDeviceToHost));
+  if (gwd) HIPCK(hipMemcpy(gwd, p->tb.gwd, sizeof(double) * p->tb.nlat, hipMemcpyDeviceToHost));
+  return EXO_OK;
+}
+
+extern "C" int exo_sht_plan_tables(exo_sht_plan_t p, double* tables) {
+  if (!p || !tables) return EXO_ERR_ARG;
+  return copy_tables(p->tb, tables);
+}
+
+static bool mode_ok(int mode) { return mode >= EXO_F64 && mode <= EXO_BF16X3; }
+
+extern "C" int exo_sht_inverse(exo_sht_plan_t p, int mode, int table, int nmem, const void* sp,
+                               long sp_stride, int src0, void* gp, long gp_stride, int dst0,
+                               int nfield, void* stream) {
+  if (!p || !mode_ok(mode) || (table != EXO_QI && table != EXO_QJ) || nmem < 0 || nfield < 0 ||
+      (nmem > 0 && nfield > 0 && (!sp || !gp)))
+    return EXO_ERR_ARG;
+  InvArgs a{};
+  a.sp = sp;
+  a.out = gp;
+  a.sp_stride = sp_stride;
+  a.out_stride = gp_stride;
+  a.nmem = nmem;
+  a.kind = INV_SCALAR;
+  a.table = table;
+  a.src0 = src0;
+  a.dst0 = dst0;
+  a.nfield = nfield;
+  a.plavor = p->tb.plavor;
+  return launch_inverse(p->tb, mode, true, a, (hipStream_t)stream);
+}
+
+extern "C" int exo_sht_inverse_uv(exo_sht_plan_t p, int mode, int nmem, const void* sp,
+                                  long sp_stride, int vor0, int div0, void* gp, long gp_stride,
+                                  int u0, int v0, int nlev, void* stream) {
+  if (!p || !mode_ok(mode) || nmem < 0 || nlev < 0 || (nmem > 0 && nlev > 0 && (!sp || !gp)))
+    return EXO_ERR_ARG;
+  InvArgs a{};
+  a.sp = sp;
+  a.out = gp;
+  a.sp_stride = sp_stride;
+  a.out_stride = gp_stride;
+  a.nmem = nmem;
+  a.kind = INV_UV;
+  a.vor0 = vor0;
+  a.div0 = div0;
+  a.u0 = u0;
+  a.v0 = v0;
+  a.nlev = nlev;
+  a.plavor = p->tb.plavor;
+  return launch_inverse(p->tb, mode, true, a, (hipStream_t)stream);
+}
+
+static int direct_impl(exo_sht_plan_t p, int mode, bool fused, int nmem, const void* in,
+                       long in_stride, int src0, void* sp, long sp_stride, int dst0, int nfield,
+                       hipStream_t s) {
+  FwdArgs a{};
+  a.in = in;
+  a.sp = sp;
+  a.in_stride = in_stride;
+  a.sp_stride = sp_stride;
+  a.nmem = nmem;
+  a.src0 = src0;
+  a.dst0 = dst0;
+  a.nfield = nfield;
+  a.table = TQC;
+  const size_t need = forward_scratch_bytes(p->tb, mode, nmem, nfield);
+  if (need > 0) {
+    int rc = grow(&p->scratch, &p->scratch_bytes, need);
+    if (rc != EXO_OK) return rc;
+  }
+  return launch_forward(p->tb, mode, fused, a, p->scratch, p->scratch_bytes, s);
+}
+
+extern "C" int exo_sht_direct(exo_sht_plan_t p, int mode, int nmem, const void* gp,
+                              long gp_stride, int src0, void* sp, long sp_stride, int dst0,
+                              int nfield, void* stream) {
+  if (!p || !mode_ok(mode) || nmem < 0 || nfield < 0 || (nmem > 0 && nfield > 0 && (!sp || !gp)))
+    return EXO_ERR_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
+  return direct_impl(p, mode, true, nmem, gp, gp_stride, src0, sp, sp_stride, dst0, nfield,
+                     (hipStream_t)stream);
+}
+
+extern "C" int exo_sht_roundtrip(exo_sht_plan_t p, int mode, int nmem, int nlev, const void* sp_in,
+                                 void* gp, void* sp_out, void* stream) {
+  if (!p || !mode_ok(mode) || nmem < 0 || nlev < 1) return EXO_ERR_ARG;
+  const Tables& t = p->tb;
+  const int nfl = 3 * nlev + 1;
+  const int ngp = nfl + 1 + 2 * nlev;
+  const long sps = (long)nfl * 2 * t.ncsp;
+  const long gps = (long)ngp * t.nlat * t.nlon;
+  int rc;
+  rc = exo_sht_inverse(p, mode, EXO_QI, nmem, sp_in, sps, 0, gp, gps, 0, nfl, stream);
+  if (rc) return rc;
+  rc = exo_sht_inverse(p, mode, EXO_QJ, nmem, sp_in, sps, nfl - 1, gp, gps, nfl, 1, stream);
+  if (rc) return rc;
+  rc = exo_sht_inverse_uv(p, mode, nmem, sp_in, sps, 0, nlev, gp, gps, nfl + 1, nfl + 1 + nlev,
+                          nlev, stream);
+  if (rc) return rc;
+  return exo_sht_direct(p, mode, nmem, gp, gps, 0, sp_out, sps, 0, nfl, stream);
+}
+
+// ---------------------------------------------------------------------------
+// drop-in entry points on the current (exo_legini) tables
+// ---------------------------------------------------------------------------
+static exo_sht_plan_s* g_plan = nullptr;
+static std::mutex g_mu;
+
+extern "C" int exo_inigau(int klat, double* pz0, double* pzw) {
+  if (klat < 2 || (klat & 1) || !pz0 || !pzw) return EXO_ERR_ARG;
+  double* d = nullptr;
+  HIPCK(hipMalloc(&d, sizeof(double) * 2 * klat));
+  k_inigau<<<(klat / 2 + 63) / 64, 64>>>(klat, d, d + klat);
+  int rc = (hipGetLastError() == hipSuccess) ? EXO_OK : EXO_ERR_LAUNCH;
+  if (rc == EXO_OK && hipMemcpy(pz0, d, sizeof(double) * klat, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = EXO_ERR_HIP;
+  if (rc == EXO_OK &&
+      hipMemcpy(pzw, d + klat, sizeof(double) * klat, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = EXO_ERR_HIP;
+  (void)hipFree(d);
+  return rc;
+}
+
+extern "C" int exo_legini(int ntru, int nlat, int nlev, const double* sid, const double* gwd,
+                          const double* skgpsp, const double* skspgp, double plavor) {
+  if (nlat_of(ntru) != nlat || nlev < 1) return EXO_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lk(g_mu);
+  exo_sht_plan_t p = nullptr;
+  int rc = exo_sht_plan_create(&p, ntru, sid, gwd, skgpsp, skspgp, plavor);
+  if (rc != EXO_OK) return rc;
+  p->nlev = nlev;
+  if (g_plan) exo_sht_plan_destroy(g_plan);
+  g_plan = p;
+  return EXO_OK;
+}
+
+extern "C" int exo_legini_tables(double* tables) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_plan) return EXO_ERR_NOINIT;
+  return copy_tables(g_plan->tb, tables);
+}
+
+// stage host arrays through device memory, run, copy back (synchronous)
+template <typename R, typename Fn>
+static int staged(size_t in_elems, const R* in, size_t out_elems, R* out, Fn run) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_plan) return EXO_ERR_NOINIT;
+  exo_sht_plan_s* p = g_plan;
+  const size_t need = sizeof(R) * (in_elems + out_elems);
+  int rc = grow(&p->stage, &p->stage_bytes, need);
+  if (rc) return rc;
+  R* din = (R*)p->stage;
+  R* dout = din + in_elems;
+  if (in_elems) HIPCK(hipMemcpy(din, in, sizeof(R) * in_elems, hipMemcpyHostToDevice));
+  rc = run(p, din, dout);
+  if (rc) return rc;
+  HIPCK(hipDeviceSynchronize());
+  if (out_elems) HIPCK(hipMemcpy(out, dout, sizeof(R) * out_elems, hipMemcpyDeviceToHost));
+  return EXO_OK;
+}
+
+template <typename R>
+static int sp2fc_impl(const R* sp, R* fc, int nfl, int table) {
+  if (!sp || !fc || nfl < 0) return EXO_ERR_ARG;
+  if (!g_plan) return EXO_ERR_NOINIT;
+  const Tables& t = g_plan->tb;
+  const long nsp = 2L * t.ncsp, nhor = (long)t.nlat * t.nlon;
+  const int mode = sizeof(R) == 8 ? MODE_F64 : MODE_F32;
+  return staged<R>(nsp * nfl, sp, nhor * nfl, fc, [&](exo_sht_plan_s* p, R* din, R* dout) {
+    InvArgs a{};
+    a.sp = din;
+    a.out = dout;
+    a.sp_stride = 0;
+    a.out_stride = 0;
+    a.nmem = 1;
+    a.kind = INV_SCALAR;
+    a.table = table;
+    a.src0 = 0;
+    a.dst0 = 0;
+    a.nfield = nfl;
+    a.plavor = p->tb.plavor;
+    return launch_inverse(p->tb, mode, false, a, 0);
+  });
+}
+
+extern "C" int exo_sp2fc_f64(const double* sp, double* fc, int nfl) {
+  return sp2fc_impl<double>(sp, fc, nfl, TQI);
+}
+extern "C" int exo_sp2fc_f32(const float* sp, float* fc, int nfl) {
+  return sp2fc_impl<float>(sp, fc, nfl, TQI);
+}
+extern "C" int exo_sp2fcdmu_f64(const double* sp, double* fc, int nfl) {
+  return sp2fc_impl<double>(sp, fc, nfl, TQJ);
+}
+extern "C" int exo_sp2fcdmu_f32(const float* sp, float* fc, int nfl) {
+  return sp2fc_impl<float>(sp, fc, nfl, TQJ);
+}
+
+template <typename R>
+static int fc2sp_impl(const R* fc, R* sp, int nfl) {
+  if (!sp || !fc || nfl < 0) return EXO_ERR_ARG;
+  if (!g_plan) return EXO_ERR_NOINIT;
+  const Tables& t = g_plan->tb;
+  const long nsp = 2L * t.ncsp, nhor = (long)t.nlat * t.nlon;
+  const int mode = sizeof(R) == 8 ? MODE_F64 : MODE_F32;
+  return staged<R>(nhor * nfl, fc, nsp * nfl, sp, [&](exo_sht_plan_s* p, R* din, R* dout) {
+    return direct_impl(p, mode, false, 1, din, 0, 0, dout, 0, 0, nfl, 0);
+  });
+}
+
+extern "C" int exo_fc2sp_f64(const double* fc, double* sp, int nfl) {
+  return fc2sp_impl<double>(fc, sp, nfl);
+}
+extern "C" int exo_fc2sp_f32(const float* fc, float* sp, int nfl) {
+  return fc2sp_impl<float>(fc, sp, nfl);
+}
+
+template <typename R>
+static int dv2uv_impl(const R* pd, const R* pz, R* pu, R* pv) {
+  if (!pd || !pz || !pu || !pv) return EXO_ERR_ARG;
+  if (!g_plan) return EXO_ERR_NOINIT;
+  const Tables& t = g_plan->tb;
+  const int nlev = g_plan->nlev;
+  const long nsp = 2L * t.ncsp, nhor = (long)t.nlat * t.nlon;
+  const int mode = sizeof(R) == 8 ? MODE_F64 : MODE_F32;
+  std::vector<R> in(2 * nsp * nlev), out(2 * nhor * nlev);
+  memcpy(in.data(), pz, sizeof(R) * nsp * nlev);
+  memcpy(in.data() + nsp * nlev, pd, sizeof(R) * nsp * nlev);
+  int rc = staged<R>(in.size(), in.data(), out.size(), out.data(),
+                     [&](exo_sht_plan_s* p, R* din, R* dout) {
+                       InvArgs a{};
+                       a.sp = din;
+                       a.out = dout;
+                       a.nmem = 1;
+                       a.kind = INV_UV;
+                       a.vor0 = 0;
+                       a.div0 = nlev;
+                       a.u0 = 0;
+                       a.v0 = nlev;
+                       a.nlev = nlev;
+                       a.plavor = p->tb.plavor;
+                       return launch_inverse(p->tb, mode, false, a, 0);
+                     });
+  if (rc) return rc;
+  memcpy(pu, out.data(), sizeof(R) * nhor * nlev);
+  memcpy(pv, out.data() + nhor * nlev, sizeof(R) * nhor * nlev);
+  return EXO_OK;
+}
+
+extern "C" int exo_dv2uv_f64(const double* pd, const double* pz, double* pu, double* pv) {
+  return dv2uv_impl<double>(pd, pz, pu, pv);
+}
+extern "C" int exo_dv2uv_f32(const float* pd, const float* pz, float* pu, float* pv) {
+  return dv2uv_impl<float>(pd, pz, pu, pv);
+}
+
+template <typename R>
+static int fft_impl(R* a, int n, int lot, int dir) {
+  if (!a || lot < 0) return EXO_ERR_ARG;
+  if (!g_plan) return EXO_ERR_NOINIT;
+  if (n != g_plan->tb.nlon) return EXO_ERR_UNSUPPORTED;
+  const size_t el = (size_t)n * lot;
+  const int mode = sizeof(R) == 8 ? MODE_F64 : MODE_F32;
+  return staged<R>(el, a, el, a, [&](exo_sht_plan_s* p, R* din, R* dout) {
+    HIPCK(hipMemcpyAsync(dout, din, sizeof(R) * el, hipMemcpyDeviceToDevice, 0));
+    return launch_fft(p->tb, mode, dir, dout, n, lot, 0);
+  });
+}
+
+extern "C" int exo_gp2fc_f64(double* a, int n, int lot) { return fft_impl<double>(a, n, lot, -1); }
+extern "C" int exo_gp2fc_f32(float* a, int n, int lot) { return fft_impl<float>(a, n, lot, -1); }
+extern "C" int exo_fc2gp_f64(double* a, int n, int lot) { return fft_impl<double>(a, n, lot, +1); }
+extern "C" int exo_fc2gp_f32(float* a, int n, int lot) { return fft_impl<float>(a, n, lot, +1); }

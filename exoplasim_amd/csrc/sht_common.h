@@ -1,0 +1,245 @@
+// sht_common.h -- shared device definitions of the MI355X spectral transform.
+//
+// Conventions (the reference's, exoplasim/plasim/src/legmod.f90 and
+// fftmod.f90):
+//   spectral  sp[2*w + ri], w = w0(m) + (n - m), m-major, n inner
+//   Fourier   fc[(lat*(NLON/2) + m)*2 + ri], lat 0..NLAT-1 north->south
+//   grid      gp[lat*NLON + j]
+// Latitude pairs: pair p holds lat p (north) and NLAT-1-p (south).  The
+// symmetric ("sym") class of a spectral mode is n-m even for P-type tables
+// (qi, qc, qm, qq, qu) and n-m odd for Q-type tables (qj, qe, qv), since
+// P(-mu) = (-1)^(n-m) P(mu) and Q(-mu) = -(-1)^(n-m) Q(mu).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace exo {
+
+enum Mode : int { MODE_F64 = 0, MODE_F32 = 1, MODE_F32MFMA = 2, MODE_BF16X3 = 3 };
+enum Table : int { TQI = 0, TQJ, TQC, TQE, TQM, TQQ, TQU, TQV, NTAB };
+
+__host__ __device__ constexpr bool table_is_q(int t) {
+  return t == TQJ || t == TQE || t == TQV;
+}
+
+// Sizes of one triangular truncation on its Gaussian grid
+// (src/plasimmod.f90:100-113 with NPRO=1).
+template <int T>
+struct TS {
+  static constexpr int NTRU = T;
+  static constexpr int NM = T + 1;
+  static constexpr int NLAT = (T == 21) ? 32 : (T == 42) ? 64 : (T == 63) ? 96 : 192;
+  static constexpr int NLON = 2 * NLAT;
+  static constexpr int NPAIR = NLAT / 2;
+  static constexpr int NCSP = (T + 1) * (T + 2) / 2;
+  static constexpr int NRSP = 2 * NCSP;
+  // K blocks of 32 spectral rows per zonal wavenumber (inverse MFMA A operand)
+  static constexpr int KB = (NM + 31) / 32;
+  // max number of modes of one symmetry class per m
+  static constexpr int KHALF = (NM + 1) / 2;
+  // row blocks of 16 per (m, class) for the direct MFMA
+  static constexpr int RB = (KHALF + 15) / 16;
+};
+
+__host__ __device__ constexpr int w0_of(int ntru, int m) {
+  return m * (ntru + 1) - m * (m - 1) / 2;
+}
+
+template <typename R>
+struct C2 {
+  R x, y;
+};
+
+template <typename R>
+__device__ __forceinline__ C2<R> cadd(C2<R> a, C2<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename R>
+__device__ __forceinline__ C2<R> csub(C2<R> a, C2<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename R>
+__device__ __forceinline__ C2<R> cmul(C2<R> a, C2<R> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// multiply by s*i (s = +1 or -1)
+template <int S, typename R>
+__device__ __forceinline__ C2<R> cmul_si(C2<R> a) {
+  return S > 0 ? C2<R>{-a.y, a.x} : C2<R>{a.y, -a.x};
+}
+
+// ---- small in-register DFTs; S = -1 forward (e^{-2 pi i/R}), +1 inverse ----
+template <int R, int S, typename F>
+struct Dft;
+
+template <int S, typename F>
+struct Dft<2, S, F> {
+  __device__ __forceinline__ static void run(C2<F>* v) {
+    C2<F> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <int S, typename F>
+struct Dft<3, S, F> {
+  __device__ __forceinline__ static void run(C2<F>* v) {
+    const F h = F(0.86602540378443864676);  // sqrt(3)/2
+    C2<F> t = cadd(v[1], v[2]);
+    C2<F> d = csub(v[1], v[2]);
+    C2<F> m = {v[0].x - F(0.5) * t.x, v[0].y - F(0.5) * t.y};
+    C2<F> q = cmul_si<S>(C2<F>{h * d.x, h * d.y});
+    v[0] = cadd(v[0], t);
+    v[1] = cadd(m, q);
+    v[2] = csub(m, q);
+  }
+};
+
+template <int S, typename F>
+struct Dft<4, S, F> {
+  __device__ __forceinline__ static void run(C2<F>* v) {
+    C2<F> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    C2<F> t2 = cadd(v[1], v[3]), t3 = cmul_si<S>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+
+template <int S, typename F>
+struct Dft<8, S, F> {
+  __device__ __forceinline__ static void run(C2<F>* v) {
+    C2<F> e[4] = {v[0], v[2], v[4], v[6]};
+    C2<F> o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4, S, F>::run(e);
+    Dft<4, S, F>::run(o);
+    const F r = F(0.70710678118654752440);
+    // w^1 = r(1 + s i), w^2 = s i, w^3 = r(-1 + s i)
+    C2<F> o1 = cmul(o[1], C2<F>{r, F(S) * r});
+    C2<F> o2 = cmul_si<S>(o[2]);
+    C2<F> o3 = cmul(o[3], C2<F>{-r, F(S) * r});
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+  }
+};
+
+// Radix plan of the row lengths used by the supported truncations.
+template <int N>
+struct FftPlan;
+template <> struct FftPlan<64>  { static constexpr int NP = 2; static constexpr int R[4] = {8, 8, 1, 1}; };
+template <> struct FftPlan<128> { static constexpr int NP = 3; static constexpr int R[4] = {8, 4, 4, 1}; };
+template <> struct FftPlan<192> { static constexpr int NP = 3; static constexpr int R[4] = {8, 8, 3, 1}; };
+template <> struct FftPlan<384> { static constexpr int NP = 4; static constexpr int R[4] = {8, 4, 4, 3}; };
+
+// FFTs in flight per pass within a 256-thread block (about 2048 points)
+template <int N>
+struct FftBatch {
+  static constexpr int NF = (N == 64) ? 32 : (N == 128) ? 16 : (N == 192) ? 10 : 5;
+};
+
+// One Stockham autosort pass (radix R, stride NS) over NF FFTs of length
+// N held in LDS.  Pass 0 reads through `load(f, k)`, the last pass writes
+// through `store(f, k, v)`; the other passes go through `buf` (NF*N
+// complex).  All 256 threads of the block must call it.  tw[k] =
+// e^{S 2 pi i k/N}.
+template <int N, int R, int NS, int S, bool FIRST, bool LAST, typename F,
+          typename Load, typename Store>
+__device__ __forceinline__ void fft_pass(C2<F>* buf, const C2<F>* tw, int nf,
+                                         Load& load, Store& store) {
+  constexpr int NB = N / R;  // butterflies per FFT
+  constexpr int BPT = (FftBatch<N>::NF * NB + 255) / 256;
+  const int tid = threadIdx.x;
+  const int total = nf * NB;
+  C2<F> v[BPT][R];
+#pragma unroll
+  for (int t = 0; t < BPT; ++t) {
+    const int b = tid + t * 256;
+    if (b < total) {
+      const int f = b / NB, j = b - f * NB;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (FIRST)
+          v[t][r] = load(f, j + r * NB);
+        else
+          v[t][r] = buf[f * N + j + r * NB];
+      }
+    }
+  }
+  if (!FIRST) __syncthreads();
+#pragma unroll
+  for (int t = 0; t < BPT; ++t) {
+    const int b = tid + t * 256;
+    if (b < total) {
+      const int f = b / NB, j = b - f * NB;
+      const int jm = j % NS;
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], tw[(jm * r * (N / (NS * R))) % N]);
+      }
+      Dft<R, S, F>::run(v[t]);
+      const int idxd = (j / NS) * NS * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (LAST)
+          store(f, idxd + r * NS, v[t][r]);
+        else
+          buf[f * N + idxd + r * NS] = v[t][r];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Full FFT of nf (<= FftBatch<N>::NF) sequences through the radix plan.
+template <int N, int S, typename F, typename Load, typename Store>
+__device__ __forceinline__ void fft_run(C2<F>* buf, const C2<F>* tw, int nf, Load& load,
+                                        Store& store) {
+  using P = FftPlan<N>;
+  constexpr int R0 = P::R[0], R1 = P::R[1], R2 = P::R[2], R3 = P::R[3];
+  if constexpr (P::NP == 2) {
+    fft_pass<N, R0, 1, S, true, false, F>(buf, tw, nf, load, store);
+    fft_pass<N, R1, R0, S, false, true, F>(buf, tw, nf, load, store);
+  } else if constexpr (P::NP == 3) {
+    fft_pass<N, R0, 1, S, true, false, F>(buf, tw, nf, load, store);
+    fft_pass<N, R1, R0, S, false, false, F>(buf, tw, nf, load, store);
+    fft_pass<N, R2, R0 * R1, S, false, true, F>(buf, tw, nf, load, store);
+  } else {
+    fft_pass<N, R0, 1, S, true, false, F>(buf, tw, nf, load, store);
+    fft_pass<N, R1, R0, S, false, false, F>(buf, tw, nf, load, store);
+    fft_pass<N, R2, R0 * R1, S, false, false, F>(buf, tw, nf, load, store);
+    fft_pass<N, R3, R0 * R1 * R2, S, false, true, F>(buf, tw, nf, load, store);
+  }
+}
+
+// ---- split-BF16 helpers ----
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t bf16_trunc_bits(float x) {
+  return (uint16_t)(__float_as_uint(x) >> 16);
+}
+
+// exact three-way split x = hi + mid + lo of an FP32 value into bf16
+// (truncation at each step, so each remainder is exact)
+__device__ __forceinline__ void split3(float x, uint16_t& h, uint16_t& m, uint16_t& l) {
+  uint32_t u = __float_as_uint(x);
+  float fh = __uint_as_float(u & 0xffff0000u);
+  float r1 = x - fh;
+  uint32_t u1 = __float_as_uint(r1);
+  float fm = __uint_as_float(u1 & 0xffff0000u);
+  float r2 = r1 - fm;
+  h = (uint16_t)(u >> 16);
+  m = (uint16_t)(u1 >> 16);
+  l = (uint16_t)(__float_as_uint(r2) >> 16);
+}
+
+union BF8 {
+  bf16x8 v;
+  uint16_t s[8];
+  uint4 q;
+};
+
+}  // namespace exo

@@ -1,0 +1,75 @@
+// sht_internal.h -- library-internal declarations (plan object, kernels,
+// launch helpers).  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "sht_common.h"
+
+namespace exo {
+
+__global__ void k_inigau(int klat, double* pz0, double* pzw);
+__global__ void k_legini(int ntru, int npair, const double* sid, const double* gwd, double* tab);
+__global__ void k_filtered(int ntru, int npair, const double* tab, const double* skgpsp,
+                           const double* skspgp, double* out64, float* out32);
+__global__ void k_pack_inv(int ntru, int npair, int kbmax, int t, const double* filt64,
+                           uint16_t* bf, float* f32);
+__global__ void k_pack_fwd(int ntru, int npair, int rbmax, int kpad, int t,
+                           const double* filt64, uint16_t* bf, float* f32);
+__global__ void k_twiddle(int n, double* tw64, float* tw32);
+
+// Which inverse transform a launch performs.
+enum InvKind : int {
+  INV_SCALAR = 0,  // sp2fc with qi (or sp2fcdmu with qj) of a run of fields
+  INV_UV = 1,      // dv2uv: (D, zeta) -> (u cos, v cos), terms qv and qu
+};
+
+// Device tables of one truncation (all FP64 values computed on the GPU).
+struct Tables {
+  int ntru = 0, nlat = 0, nlon = 0, npair = 0, ncsp = 0;
+  int kbmax = 0, rbmax = 0, kpad = 0;
+  double* sid = nullptr;      // [nlat]
+  double* gwd = nullptr;      // [nlat]
+  double* tab = nullptr;      // [8][npair][ncsp] unfiltered
+  double* filt64 = nullptr;   // [8][npair][ncsp] filter folded in
+  float* filt32 = nullptr;
+  double* skgpsp = nullptr;   // [ntru+1]
+  double* skspgp = nullptr;
+  uint16_t* inv_bf[NTAB] = {};  // packed inverse A operands, 3 planes
+  float* inv_f32[NTAB] = {};
+  uint16_t* fwd_bf[NTAB] = {};  // packed direct A operands, 3 planes
+  float* fwd_f32[NTAB] = {};
+  long inv_elems = 0, fwd_elems = 0;
+  double* tw64 = nullptr;     // [nlon] complex
+  float* tw32 = nullptr;
+  double plavor = 0.0;
+};
+
+// Launch arguments of the fused/unfused inverse and direct transforms.
+struct InvArgs {
+  const void* sp;   // spectral input (member stride sp_stride reals)
+  void* out;        // grid (fused) or Fourier (unfused) output
+  long sp_stride, out_stride;
+  int nmem;
+  int kind;
+  int table;        // INV_SCALAR: TQI or TQJ
+  int src0, dst0, nfield;               // INV_SCALAR field runs
+  int vor0, div0, u0, v0, nlev;         // INV_UV
+  double plavor;
+};
+
+struct FwdArgs {
+  const void* in;   // grid (fused) or Fourier (unfused) input
+  void* sp;         // spectral output (or partial sums when npb > 1)
+  long in_stride, sp_stride;
+  int nmem;
+  int src0, dst0, nfield;
+  int table;        // TQC
+};
+
+int launch_inverse(const Tables& T, int mode, bool fused, const InvArgs& a, hipStream_t s);
+int launch_forward(const Tables& T, int mode, bool fused, const FwdArgs& a, void* scratch,
+                   size_t scratch_bytes, hipStream_t s);
+size_t forward_scratch_bytes(const Tables& T, int mode, int nmem, int nfield);
+int launch_fft(const Tables& T, int mode, int dir, void* a, int n, long lot, hipStream_t s);
+
+}  // namespace exo

@@ -1,0 +1,274 @@
+"""Host-side mirror of the reference's spectral-transform interface on top of
+libexosht.so.
+
+Two layers, as in include/exo_sht.h:
+
+* legmod / fftmod / gaussmod-named functions (``inigau``, ``legini``,
+  ``sp2fc``, ``sp2fcdmu``, ``fc2sp``, ``dv2uv``, ``gp2fc``, ``fc2gp``) taking
+  numpy arrays in the reference layout (exoplasim/plasim/src/legmod.f90,
+  fftmod.f90, gaussmod.f90); float64 arrays use the FP64 path, float32 the
+  FP32 path.  ``legini`` must be called first, as in the model
+  (src/plasim.f90:183,411).
+* ``Plan``: a device-resident batched transform of one truncation working on
+  torch tensors that already live on the GPU; the caller's current torch
+  stream is used.
+
+Everything runs on the GPU through the HIP library; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, byref, c_double, c_float, c_int, c_void_p
+
+import numpy as np
+
+from ._lib import ExoError, check, lib
+
+PLAVOR = 1.63299310207  # src/plasimmod.f90:118,227 (plavor = EZ)
+MODES = {"f64": 0, "f32": 1, "f32mfma": 2, "bf16x3": 3}
+TABLES = ("qi", "qj", "qc", "qe", "qm", "qq", "qu", "qv")
+NLAT_OF = {21: 32, 42: 64, 63: 96, 127: 192}
+
+_dp = POINTER(c_double)
+_fp = POINTER(c_float)
+
+
+def _cd(a: np.ndarray):
+    return a.ctypes.data_as(_dp if a.dtype == np.float64 else _fp)
+
+
+def _opt(a):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a
+
+
+def mode_id(mode) -> int:
+    if isinstance(mode, int):
+        return mode
+    return MODES[mode]
+
+
+def dims(ntru: int):
+    nlat = NLAT_OF[ntru]
+    return nlat, 2 * nlat, (ntru + 1) * (ntru + 2) // 2
+
+
+# ---------------------------------------------------------------------------
+# drop-in layer (host arrays)
+# ---------------------------------------------------------------------------
+_state = {"ntru": None, "nlat": None, "nlev": None}
+
+
+def inigau(klat: int):
+    """Gaussian abscissas (north->south) and weights (src/gaussmod.f90:5)."""
+    pz0 = np.zeros(klat)
+    pzw = np.zeros(klat)
+    check(lib().exo_inigau(klat, _cd(pz0), _cd(pzw)), "inigau")
+    return pz0, pzw
+
+
+def legini(ntru: int, nlat: int, nlev: int, sid=None, gwd=None, skgpsp=None, skspgp=None,
+           plavor: float = PLAVOR) -> None:
+    """Build the Legendre tables on the GPU (src/legmod.f90:30)."""
+    keep = [_opt(x) for x in (sid, gwd, skgpsp, skspgp)]
+    ptrs = [None if x is None else _cd(x) for x in keep]
+    check(lib().exo_legini(ntru, nlat, nlev, *ptrs, plavor), "legini")
+    _state.update(ntru=ntru, nlat=nlat, nlev=nlev)
+
+
+def legini_tables() -> np.ndarray:
+    nlat, _, ncsp = dims(_need())
+    t = np.zeros((8, nlat, ncsp))
+    check(lib().exo_legini_tables(_cd(t)), "legini_tables")
+    return t
+
+
+def _need() -> int:
+    if _state["ntru"] is None:
+        raise ExoError("legini has not been called")
+    return _state["ntru"]
+
+
+def _suffix(a: np.ndarray) -> str:
+    if a.dtype == np.float64:
+        return "f64"
+    if a.dtype == np.float32:
+        return "f32"
+    raise TypeError("float64 or float32 arrays expected")
+
+
+def _call(name, *arrs, extra=()):
+    fn = getattr(lib(), f"exo_{name}_{_suffix(arrs[0])}")
+    check(fn(*[_cd(a) for a in arrs], *extra), name)
+
+
+def sp2fc(sp: np.ndarray) -> np.ndarray:
+    """sp(2,NCSP[,nfl]) -> fc(2,NLON/2,NLAT[,nfl]) (src/legmod.f90:230)."""
+    ntru = _need()
+    nlat, nlon, ncsp = dims(ntru)
+    sp = np.ascontiguousarray(sp)
+    nfl = sp.size // (2 * ncsp)
+    fc = np.zeros((nfl, nlat, nlon), dtype=sp.dtype)
+    _call("sp2fc", sp, fc, extra=(nfl,))
+    return fc
+
+
+def sp2fcdmu(sp: np.ndarray) -> np.ndarray:
+    """Fourier coefficients of (1-mu^2) d/dmu (src/legmod.f90:262)."""
+    ntru = _need()
+    nlat, nlon, ncsp = dims(ntru)
+    sp = np.ascontiguousarray(sp)
+    nfl = sp.size // (2 * ncsp)
+    fc = np.zeros((nfl, nlat, nlon), dtype=sp.dtype)
+    _call("sp2fcdmu", sp, fc, extra=(nfl,))
+    return fc
+
+
+def fc2sp(fc: np.ndarray) -> np.ndarray:
+    """fc(2,NLON/2,NLAT[,nfl]) -> sp(2,NCSP[,nfl]) (src/legmod.f90:178)."""
+    ntru = _need()
+    nlat, nlon, ncsp = dims(ntru)
+    fc = np.ascontiguousarray(fc)
+    nfl = fc.size // (nlat * nlon)
+    sp = np.zeros((nfl, 2 * ncsp), dtype=fc.dtype)
+    _call("fc2sp", fc, sp, extra=(nfl,))
+    return sp
+
+
+def dv2uv(pd: np.ndarray, pz: np.ndarray):
+    """(divergence, absolute vorticity) -> (u cos, v cos) Fourier
+    coefficients over NLEV levels (src/legmod.f90:312)."""
+    ntru = _need()
+    nlat, nlon, ncsp = dims(ntru)
+    nlev = _state["nlev"]
+    pd = np.ascontiguousarray(pd)
+    pz = np.ascontiguousarray(pz, dtype=pd.dtype)
+    pu = np.zeros((nlev, nlat, nlon), dtype=pd.dtype)
+    pv = np.zeros((nlev, nlat, nlon), dtype=pd.dtype)
+    _call("dv2uv", pd, pz, pu, pv)
+    return pu, pv
+
+
+def gp2fc(a: np.ndarray) -> np.ndarray:
+    """Forward real FFT of rows, returns the transformed copy
+    (src/fftmod.f90:29)."""
+    a = np.array(a, order="C")
+    n = a.shape[-1]
+    _call("gp2fc", a, extra=(n, a.size // n))
+    return a
+
+
+def fc2gp(a: np.ndarray) -> np.ndarray:
+    """Inverse real FFT of rows, returns the transformed copy
+    (src/fftmod.f90:64)."""
+    a = np.array(a, order="C")
+    n = a.shape[-1]
+    _call("fc2gp", a, extra=(n, a.size // n))
+    return a
+
+
+# ---------------------------------------------------------------------------
+# device-resident batched layer
+# ---------------------------------------------------------------------------
+def _ptr(t) -> int:
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ExoError("device tensors expected")
+    if not t.is_contiguous():
+        raise ExoError("contiguous tensors expected")
+    return t.data_ptr()
+
+
+def _stream():
+    import torch
+
+    return torch.cuda.current_stream().cuda_stream
+
+
+class Plan:
+    """Transforms of one truncation with FP64 tables built on the GPU."""
+
+    def __init__(self, ntru: int, sid=None, gwd=None, skgpsp=None, skspgp=None,
+                 plavor: float = PLAVOR):
+        keep = [_opt(x) for x in (sid, gwd, skgpsp, skspgp)]
+        ptrs = [None if x is None else _cd(x) for x in keep]
+        h = c_void_p()
+        check(lib().exo_sht_plan_create(byref(h), ntru, *ptrs, plavor), "plan_create")
+        self._h = h
+        d = (c_int * 4)()
+        check(lib().exo_sht_plan_dims(h, d), "plan_dims")
+        self.ntru, self.nlat, self.nlon, self.ncsp = list(d)
+        self.nrsp = 2 * self.ncsp
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().exo_sht_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def grid(self):
+        sid = np.zeros(self.nlat)
+        gwd = np.zeros(self.nlat)
+        check(lib().exo_sht_plan_grid(self._h, _cd(sid), _cd(gwd)), "plan_grid")
+        return sid, gwd
+
+    def tables(self) -> np.ndarray:
+        t = np.zeros((8, self.nlat, self.ncsp))
+        check(lib().exo_sht_plan_tables(self._h, _cd(t)), "plan_tables")
+        return t
+
+    @staticmethod
+    def dtype_of(mode):
+        import torch
+
+        return torch.float64 if mode_id(mode) == 0 else torch.float32
+
+    def inverse(self, mode, sp, gp, nfield, src0=0, dst0=0, table="qi", nmem=None):
+        """sp2fc (table qi) or sp2fcdmu (qj) + fc2gp of field-levels
+        [src0, src0+nfield) of every member into grid fields [dst0, ...).
+        sp: (nmem, nfl, nrsp), gp: (nmem, ngp, nlat, nlon)."""
+        nmem = sp.shape[0] if nmem is None else nmem
+        check(lib().exo_sht_inverse(self._h, mode_id(mode), TABLES.index(table), nmem,
+                                    _ptr(sp), sp[0].numel(), src0, _ptr(gp), gp[0].numel(),
+                                    dst0, nfield, _stream()), "inverse")
+
+    def inverse_uv(self, mode, sp, gp, nlev, vor0, div0, u0, v0, nmem=None):
+        nmem = sp.shape[0] if nmem is None else nmem
+        check(lib().exo_sht_inverse_uv(self._h, mode_id(mode), nmem, _ptr(sp), sp[0].numel(),
+                                       vor0, div0, _ptr(gp), gp[0].numel(), u0, v0, nlev,
+                                       _stream()), "inverse_uv")
+
+    def direct(self, mode, gp, sp, nfield, src0=0, dst0=0, nmem=None):
+        """gp2fc + fc2sp of grid fields [src0, src0+nfield) into spectral
+        field-levels [dst0, ...)."""
+        nmem = gp.shape[0] if nmem is None else nmem
+        check(lib().exo_sht_direct(self._h, mode_id(mode), nmem, _ptr(gp), gp[0].numel(),
+                                   src0, _ptr(sp), sp[0].numel(), dst0, nfield, _stream()),
+              "direct")
+
+    def roundtrip(self, mode, sp_in, gp, sp_out, nlev):
+        """One round trip of every member (see exo_sht_roundtrip)."""
+        check(lib().exo_sht_roundtrip(self._h, mode_id(mode), sp_in.shape[0], nlev,
+                                      _ptr(sp_in), _ptr(gp), _ptr(sp_out), _stream()),
+              "roundtrip")
+
+    def alloc_roundtrip(self, nmem: int, nlev: int, mode):
+        """Zeroed device buffers (sp_in, gp, sp_out) of a round trip."""
+        import torch
+
+        dt = self.dtype_of(mode)
+        nfl = 3 * nlev + 1
+        ngp = nfl + 1 + 2 * nlev
+        dev = torch.device("cuda")
+        sp_in = torch.zeros((nmem, nfl, self.nrsp), dtype=dt, device=dev)
+        gp = torch.zeros((nmem, ngp, self.nlat, self.nlon), dtype=dt, device=dev)
+        sp_out = torch.zeros((nmem, nfl, self.nrsp), dtype=dt, device=dev)
+        return sp_in, gp, sp_out

@@ -1,0 +1,146 @@
+/*
+ * exo_sht.h -- C ABI of the MI355X spectral-transform library (libexosht.so).
+ *
+ * Two layers:
+ *
+ * 1. Drop-in entry points with the argument meaning of the reference's
+ *    legmod / fftmod / gaussmod subroutines (exoplasim/plasim/src/).  They
+ *    take host arrays in the reference layout, run on the GPU and return when
+ *    the result is back in the caller's arrays.  The _f64 / _f32 suffix is
+ *    the caller's `real` kind (MOST_PREC 8 or 4).  The Fortran ISO_C_BINDING
+ *    shim (exoplasim_amd/fortran/exo_legmod_shim.f90) exports the original
+ *    subroutine names on top of them.
+ *
+ * 2. A device-resident batched API on plans: many independent members
+ *    transformed per launch with inputs and outputs already in device memory,
+ *    enqueued on a caller stream.  This is what the throughput benchmark
+ *    measures.
+ *
+ * Layouts (reference, Fortran column-major):
+ *   spectral  sp[2*w + {0,1}]  (re, im), w = m-major, n inner, NRSP reals
+ *   Fourier   fc[(lat*(NLON/2) + m)*2 + {0,1}], lat north -> south
+ *   grid      gp[lat*NLON + j]  (the same memory as the Fourier array)
+ * Every entry point returns EXO_OK (0) or a negative EXO_ERR_* code; the
+ * Fortran shim turns a non-zero code into the model's abort path.
+ */
+#ifndef EXO_SHT_H
+#define EXO_SHT_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EXO_OK 0
+#define EXO_ERR_ARG (-1)
+#define EXO_ERR_UNSUPPORTED (-2)
+#define EXO_ERR_LAUNCH (-3)
+#define EXO_ERR_SCRATCH (-4)
+#define EXO_ERR_HIP (-5)
+#define EXO_ERR_NOINIT (-6)
+
+/* Legendre arithmetic of a transform call */
+#define EXO_F64 0     /* FP64 vector FMA; FP64 data and FFT (the plain path) */
+#define EXO_F32 1     /* FP32 vector FMA; FP32 data and FFT                 */
+#define EXO_F32MFMA 2 /* v_mfma_f32_16x16x4_f32, FP32 data and FFT          */
+#define EXO_BF16X3 3  /* split-BF16 x3 on v_mfma_f32_16x16x32_bf16, FP32 acc */
+
+/* legmod table ids (src/legmod.f90:13-20) */
+#define EXO_QI 0
+#define EXO_QJ 1
+#define EXO_QC 2
+#define EXO_QE 3
+#define EXO_QM 4
+#define EXO_QQ 5
+#define EXO_QU 6
+#define EXO_QV 7
+
+const char *exo_strerror(int code);
+int exo_device_count(void);
+
+/* ====================================================================== */
+/* 1. drop-in entry points (host arrays, synchronous)                      */
+/* ====================================================================== */
+
+/* inigau (src/gaussmod.f90:5): Gaussian abscissas sin(phi) and weights,
+ * north to south, computed in FP64 on the GPU. */
+int exo_inigau(int klat, double *pz0, double *pzw);
+
+/* legini (src/legmod.f90:30): builds the FP64 Legendre tables on the GPU
+ * from the model's sid/gwd (src/plasimmod.f90:577-578) and makes them the
+ * current tables of the drop-in entry points.  sid/gwd may be NULL (then
+ * computed by inigau on the GPU); skgpsp/skspgp (NTRU+1 values each,
+ * src/legmod.f90:21-22) may be NULL for no filter.  nlev is the model's
+ * NLEV (used by dv2uv/uv2dv/mktend/qtend).  Supported truncations:
+ * 21, 42, 63, 127 with nlat = 32, 64, 96, 192. */
+int exo_legini(int ntru, int nlat, int nlev, const double *sid, const double *gwd,
+               const double *skgpsp, const double *skspgp, double plavor);
+/* copy of the current FP64 tables, tables[8][nlat][ncsp] in the order
+ * qi qj qc qe qm qq qu qv (southern half by symmetry) */
+int exo_legini_tables(double *tables);
+
+/* sp2fc (src/legmod.f90:230): sp(2,NCSP) -> fc(2,NLON/2,NLAT), nfl
+ * consecutive field-levels (NRSP / NLAT*NLON reals apart) */
+int exo_sp2fc_f64(const double *sp, double *fc, int nfl);
+int exo_sp2fc_f32(const float *sp, float *fc, int nfl);
+/* sp2fcdmu (src/legmod.f90:262) */
+int exo_sp2fcdmu_f64(const double *sp, double *fc, int nfl);
+int exo_sp2fcdmu_f32(const float *sp, float *fc, int nfl);
+/* fc2sp (src/legmod.f90:178), single-process folded branch */
+int exo_fc2sp_f64(const double *fc, double *sp, int nfl);
+int exo_fc2sp_f32(const float *fc, float *sp, int nfl);
+/* dv2uv (src/legmod.f90:312): pd, pz (NRSP x NLEV), pu, pv (NHOR x NLEV);
+ * pz is absolute vorticity, plavor is subtracted from its (m=0,n=1) real
+ * coefficient (the caller's array is not modified) */
+int exo_dv2uv_f64(const double *pd, const double *pz, double *pu, double *pv);
+int exo_dv2uv_f32(const float *pd, const float *pz, float *pu, float *pv);
+/* gp2fc / fc2gp (src/fftmod.f90:29,64), in place over lot rows of length n */
+int exo_gp2fc_f64(double *a, int n, int lot);
+int exo_gp2fc_f32(float *a, int n, int lot);
+int exo_fc2gp_f64(double *a, int n, int lot);
+int exo_fc2gp_f32(float *a, int n, int lot);
+
+/* ====================================================================== */
+/* 2. device-resident batched API                                          */
+/* ====================================================================== */
+
+typedef struct exo_sht_plan_s *exo_sht_plan_t;
+
+/* plan of one truncation; arguments as exo_legini */
+int exo_sht_plan_create(exo_sht_plan_t *plan, int ntru, const double *sid, const double *gwd,
+                        const double *skgpsp, const double *skspgp, double plavor);
+int exo_sht_plan_destroy(exo_sht_plan_t plan);
+/* dims = {ntru, nlat, nlon, ncsp} */
+int exo_sht_plan_dims(exo_sht_plan_t plan, int *dims);
+int exo_sht_plan_grid(exo_sht_plan_t plan, double *sid, double *gwd);
+int exo_sht_plan_tables(exo_sht_plan_t plan, double *tables);
+
+/* Batched inverse transform fused with fc2gp: for each of nmem members,
+ * field-levels [src0, src0+nfield) of sp (member stride sp_stride reals) ->
+ * grid fields [dst0, ...) of gp (member stride gp_stride reals).  table is
+ * EXO_QI (sp2fc) or EXO_QJ (sp2fcdmu).  Device pointers, element type
+ * double for EXO_F64 else float; asynchronous on stream (hipStream_t, NULL
+ * = default stream). */
+int exo_sht_inverse(exo_sht_plan_t plan, int mode, int table, int nmem, const void *sp,
+                    long sp_stride, int src0, void *gp, long gp_stride, int dst0, int nfield,
+                    void *stream);
+/* Batched dv2uv fused with fc2gp: vorticity levels vor0.., divergence
+ * levels div0.. -> u*cos levels u0.., v*cos levels v0.. */
+int exo_sht_inverse_uv(exo_sht_plan_t plan, int mode, int nmem, const void *sp, long sp_stride,
+                       int vor0, int div0, void *gp, long gp_stride, int u0, int v0, int nlev,
+                       void *stream);
+/* Batched gp2fc fused with fc2sp (folded): grid fields [src0, ...) ->
+ * spectral field-levels [dst0, ...). */
+int exo_sht_direct(exo_sht_plan_t plan, int mode, int nmem, const void *gp, long gp_stride,
+                   int src0, void *sp, long sp_stride, int dst0, int nfield, void *stream);
+/* One round trip of nmem members with nlev levels (the benchmark unit):
+ * sp_in holds nfl = 3*nlev+1 field-levels per member (vorticity, divergence,
+ * temperature levels, then ln ps); gp receives nfl + 1 + 2*nlev grid fields
+ * per member (the nfl fields, d(ln ps)/dmu, u*cos, v*cos); sp_out receives
+ * the nfl spectral field-levels of gp2fc+fc2sp of the first nfl grid
+ * fields. */
+int exo_sht_roundtrip(exo_sht_plan_t plan, int mode, int nmem, int nlev, const void *sp_in,
+                      void *gp, void *sp_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,346 @@
+"""Python access to the oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loads ``liboracle.so`` (the FP64 C restatement of the reference's spectral
+transform core, oracle/exo_oracle.c) and, when they were built in this tree,
+the reference's own Fortran compiled into ``oracle/_ref/``.  Only tests/,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of bench.py import
+this module; the product package never does.
+
+Array conventions are the reference's (Fortran column-major, first index
+fastest); see exo_oracle.h.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, byref, c_double, c_int, c_long
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(HERE, "_ref")
+
+_dp = POINTER(c_double)
+
+# order of the eight legmod tables (src/legmod.f90:13-20 as stored by legini)
+TABLES = ("qi", "qj", "qc", "qe", "qm", "qq", "qu", "qv")
+# plavor as the model holds it (src/plasimmod.f90:118,227: plavor = EZ)
+PLAVOR = 1.63299310207
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def dims(ntru: int):
+    nlat = 3 * ntru // 2 + 1  # NTRU = (NLON-1)/3 with NLON = 2 NLAT
+    nlat = {21: 32, 42: 64, 63: 96, 85: 128, 127: 192, 170: 256}.get(ntru, nlat)
+    nlon = 2 * nlat
+    ncsp = (ntru + 1) * (ntru + 2) // 2
+    return nlat, nlon, ncsp
+
+
+class Oracle:
+    """ctypes wrapper of liboracle.so."""
+
+    def __init__(self, path: str | None = None):
+        path = path or os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (run `make -C oracle`)")
+        self.lib = L = ctypes.CDLL(path)
+        L.ora_ql.restype = c_double
+        L.ora_ql.argtypes = [c_int, c_double]
+        L.ora_qld.restype = c_double
+        L.ora_qld.argtypes = [c_int, c_double]
+        L.ora_roundtrip_worksize.restype = c_long
+        for name in ("ora_gp2fc", "ora_fc2gp", "ora_roundtrip", "ora_fft_supported",
+                     "ora_minvers"):
+            getattr(L, name).restype = c_int
+
+    # -- grid and tables -------------------------------------------------
+    def inigau(self, nlat: int):
+        sid = np.zeros(nlat)
+        gwd = np.zeros(nlat)
+        self.lib.ora_inigau(c_int(nlat), _p(sid), _p(gwd))
+        return sid, gwd
+
+    def legini(self, ntru: int, nlat: int, sid=None, gwd=None) -> dict:
+        if sid is None:
+            sid, gwd = self.inigau(nlat)
+        ncsp = (ntru + 1) * (ntru + 2) // 2
+        tab = np.zeros((8, nlat, ncsp))
+        self.lib.ora_legini(c_int(ntru), c_int(nlat), _p(np.ascontiguousarray(sid)),
+                            _p(np.ascontiguousarray(gwd)), _p(tab))
+        return {k: tab[i] for i, k in enumerate(TABLES)} | {"_all": tab}
+
+    def filters(self, ntru, nfilter=0, ngptfilter=0, nspvfilter=0,
+                filterkappa=8.0, nfilterexp=8, landhoskn0=15.0):
+        g = np.zeros(ntru + 1)
+        s = np.zeros(ntru + 1)
+        self.lib.ora_filters(c_int(ntru), c_int(nfilter), c_int(ngptfilter),
+                             c_int(nspvfilter), c_double(filterkappa),
+                             c_int(nfilterexp), c_double(landhoskn0), _p(g), _p(s))
+        return g, s
+
+    # -- transforms (single field-level unless noted) --------------------
+    def sp2fc(self, ntru, nlat, nlon, q, sk, sp):
+        fc = np.zeros((nlat, nlon))
+        self.lib.ora_sp2fc(c_int(ntru), c_int(nlat), c_int(nlon),
+                           _p(np.ascontiguousarray(q)), _p(sk),
+                           _p(np.ascontiguousarray(sp)), _p(fc))
+        return fc
+
+    def fc2sp(self, ntru, nlat, nlon, qc, sk, fc):
+        ncsp = (ntru + 1) * (ntru + 2) // 2
+        sp = np.zeros(2 * ncsp)
+        self.lib.ora_fc2sp(c_int(ntru), c_int(nlat), c_int(nlon),
+                           _p(np.ascontiguousarray(qc)), _p(sk),
+                           _p(np.ascontiguousarray(fc)), _p(sp))
+        return sp
+
+    def dv2uv(self, ntru, nlat, nlon, nlev, qu, qv, sk, plavor, pd, pz):
+        pu = np.zeros((nlev, nlat, nlon))
+        pv = np.zeros((nlev, nlat, nlon))
+        self.lib.ora_dv2uv(c_int(ntru), c_int(nlat), c_int(nlon), c_int(nlev),
+                           _p(np.ascontiguousarray(qu)), _p(np.ascontiguousarray(qv)),
+                           _p(sk), c_double(plavor), _p(np.ascontiguousarray(pd)),
+                           _p(np.ascontiguousarray(pz)), _p(pu), _p(pv))
+        return pu, pv
+
+    def uv2dv(self, ntru, nlat, nlon, nlev, qe, qm, sk, pu, pv):
+        ncsp = (ntru + 1) * (ntru + 2) // 2
+        pd = np.zeros((nlev, 2 * ncsp))
+        pz = np.zeros((nlev, 2 * ncsp))
+        self.lib.ora_uv2dv(c_int(ntru), c_int(nlat), c_int(nlon), c_int(nlev),
+                           _p(np.ascontiguousarray(qe)), _p(np.ascontiguousarray(qm)),
+                           _p(sk), _p(np.ascontiguousarray(pu)),
+                           _p(np.ascontiguousarray(pv)), _p(pd), _p(pz))
+        return pd, pz
+
+    def qtend(self, ntru, nlat, nlon, nlev, qc, qe, qm, sk, qn, uq, vq):
+        ncsp = (ntru + 1) * (ntru + 2) // 2
+        q = np.zeros((nlev, 2 * ncsp))
+        c = np.ascontiguousarray
+        self.lib.ora_qtend(c_int(ntru), c_int(nlat), c_int(nlon), c_int(nlev),
+                           _p(c(qc)), _p(c(qe)), _p(c(qm)), _p(sk), _p(c(qn)),
+                           _p(c(uq)), _p(c(vq)), _p(q))
+        return q
+
+    def mktend(self, ntru, nlat, nlon, nlev, qc, qe, qm, qq, sk, tn, fu, fv, ke,
+               ut, vt):
+        ncsp = (ntru + 1) * (ntru + 2) // 2
+        d = np.zeros((nlev, 2 * ncsp))
+        t = np.zeros((nlev, 2 * ncsp))
+        z = np.zeros((nlev, 2 * ncsp))
+        c = np.ascontiguousarray
+        self.lib.ora_mktend(c_int(ntru), c_int(nlat), c_int(nlon), c_int(nlev),
+                            _p(c(qc)), _p(c(qe)), _p(c(qm)), _p(c(qq)), _p(sk),
+                            _p(c(tn)), _p(c(fu)), _p(c(fv)), _p(c(ke)), _p(c(ut)),
+                            _p(c(vt)), _p(d), _p(t), _p(z))
+        return d, t, z
+
+    def gp2fc(self, a: np.ndarray, dft: bool = False) -> np.ndarray:
+        a = np.array(a, dtype=np.float64, order="C")
+        n = a.shape[-1]
+        lot = a.size // n
+        if dft:
+            self.lib.ora_gp2fc_dft(_p(a), c_int(n), c_int(lot))
+        elif self.lib.ora_gp2fc(_p(a), c_int(n), c_int(lot)) != 0:
+            raise ValueError(f"fftmod does not support n={n}")
+        return a
+
+    def fc2gp(self, a: np.ndarray, dft: bool = False) -> np.ndarray:
+        a = np.array(a, dtype=np.float64, order="C")
+        n = a.shape[-1]
+        lot = a.size // n
+        if dft:
+            self.lib.ora_fc2gp_dft(_p(a), c_int(n), c_int(lot))
+        elif self.lib.ora_fc2gp(_p(a), c_int(n), c_int(lot)) != 0:
+            raise ValueError(f"fftmod does not support n={n}")
+        return a
+
+    def roundtrip(self, ntru, nlev, sp_in, tables=None, sk=None, use_dft=None,
+                  plavor=PLAVOR):
+        """One member round trip; returns (gp, sp_out)."""
+        nlat, nlon, ncsp = dims(ntru)
+        if tables is None:
+            tables = self.legini(ntru, nlat)["_all"]
+        if sk is None:
+            sk = (np.ones(ntru + 1), np.ones(ntru + 1))
+        if use_dft is None:
+            use_dft = not bool(self.lib.ora_fft_supported(c_int(nlon)))
+        nfl = 3 * nlev + 1
+        ngp = nfl + 1 + 2 * nlev
+        sp_in = np.ascontiguousarray(sp_in, dtype=np.float64).reshape(nfl, 2 * ncsp)
+        gp = np.zeros((ngp, nlat, nlon))
+        sp_out = np.zeros((nfl, 2 * ncsp))
+        work = np.zeros(self.lib.ora_roundtrip_worksize(ntru, nlat, nlon, nlev))
+        rc = self.lib.ora_roundtrip(c_int(ntru), c_int(nlat), c_int(nlon), c_int(nlev),
+                                    _p(np.ascontiguousarray(tables)), _p(sk[1]),
+                                    _p(sk[0]), c_double(plavor), c_int(int(use_dft)),
+                                    _p(sp_in), _p(gp), _p(sp_out), _p(work))
+        if rc != 0:
+            raise ValueError("oracle round trip failed")
+        return gp, sp_out
+
+    def minvers(self, a: np.ndarray) -> np.ndarray:
+        b = np.asfortranarray(a, dtype=np.float64).copy(order="F")
+        flat = np.ascontiguousarray(b.T)  # column-major buffer
+        self.lib.ora_minvers(_p(flat), c_int(a.shape[0]))
+        return flat.T.copy()
+
+    def makebm(self, nlev, ntru, delt, t0, dsigma, g, tau):
+        """g, tau given as (nlev, nlev) numpy arrays in math (row, col) order."""
+        gcm = np.ascontiguousarray(np.asarray(g, dtype=np.float64).T)
+        tcm = np.ascontiguousarray(np.asarray(tau, dtype=np.float64).T)
+        bm1 = np.zeros((ntru, nlev, nlev))
+        self.lib.ora_makebm(c_int(nlev), c_int(ntru), c_double(delt),
+                            _p(np.ascontiguousarray(t0, dtype=np.float64)),
+                            _p(np.ascontiguousarray(dsigma, dtype=np.float64)),
+                            _p(gcm), _p(tcm), _p(bm1))
+        return np.transpose(bm1, (0, 2, 1)).copy()  # bm1[jn][row][col]
+
+
+# ---------------------------------------------------------------------------
+# the reference's own Fortran, compiled into oracle/_ref (optional)
+# ---------------------------------------------------------------------------
+
+def _ref_path(name: str) -> str:
+    return os.path.join(REF_DIR, name)
+
+
+def ref_available(name: str = "libref_t21.so") -> bool:
+    return os.path.exists(_ref_path(name))
+
+
+class RefT21:
+    """The reference legmod/gaussmod/fftmod at its default resolution module
+    (T21 L10, src/resmod_def.f90), FP64."""
+
+    def __init__(self):
+        self.lib = L = ctypes.CDLL(_ref_path("libref_t21.so"))
+        d = (c_int * 6)()
+        L.ref_t21_dims(d)
+        self.nlat, self.nlon, self.ntru, self.nlev, self.ncsp, self.nesp = list(d)
+        self.sid = np.zeros(self.nlat)
+        self.gwd = np.zeros(self.nlat)
+        L.ref_t21_init(_p(self.sid), _p(self.gwd))
+        tab = np.zeros((8, self.nlat, self.ncsp))
+        L.ref_t21_tables(_p(tab))
+        self.tables = tab
+        pl = c_double()
+        L.ref_t21_plavor(byref(pl))
+        self.plavor = pl.value
+
+    def _fc(self, nlev=1):
+        return np.zeros((nlev, self.nlat, self.nlon))
+
+    def sp2fc(self, sp):
+        fc = self._fc()[0]
+        self.lib.sp2fc_(_p(np.ascontiguousarray(sp)), _p(fc))
+        return fc
+
+    def sp2fcdmu(self, sp):
+        fc = self._fc()[0]
+        self.lib.sp2fcdmu_(_p(np.ascontiguousarray(sp)), _p(fc))
+        return fc
+
+    def fc2sp(self, fc):
+        sp = np.zeros(self.nesp)
+        self.lib.fc2sp_(_p(np.ascontiguousarray(fc)), _p(sp))
+        return sp
+
+    def dv2uv(self, pd, pz):
+        pu, pv = self._fc(self.nlev), self._fc(self.nlev)
+        pz = np.array(pz, dtype=np.float64)
+        self.lib.dv2uv_(_p(np.ascontiguousarray(pd)), _p(pz), _p(pu), _p(pv))
+        return pu, pv
+
+    def uv2dv(self, pu, pv):
+        pd = np.zeros((self.nlev, self.nesp))
+        pz = np.zeros((self.nlev, self.nesp))
+        self.lib.uv2dv_(_p(np.ascontiguousarray(pu)), _p(np.ascontiguousarray(pv)),
+                        _p(pd), _p(pz))
+        return pd, pz
+
+    def qtend(self, qn, uq, vq):
+        q = np.zeros((self.nlev, self.nesp))
+        c = np.ascontiguousarray
+        self.lib.qtend_(_p(q), _p(c(qn)), _p(c(uq)), _p(c(vq)))
+        return q
+
+    def mktend(self, tn, fu, fv, ke, ut, vt):
+        d, t, z = (np.zeros((self.nlev, self.nesp)) for _ in range(3))
+        c = np.ascontiguousarray
+        self.lib.mktend_(_p(d), _p(t), _p(z), _p(c(tn)), _p(c(fu)), _p(c(fv)),
+                         _p(c(ke)), _p(c(ut)), _p(c(vt)))
+        return d, t, z
+
+    def gp2fc(self, a):
+        a = np.array(a, dtype=np.float64, order="C")
+        n = a.shape[-1]
+        self.lib.gp2fc_(_p(a), byref(c_int(n)), byref(c_int(a.size // n)))
+        return a
+
+    def fc2gp(self, a):
+        a = np.array(a, dtype=np.float64, order="C")
+        n = a.shape[-1]
+        self.lib.fc2gp_(_p(a), byref(c_int(n)), byref(c_int(a.size // n)))
+        return a
+
+
+class RefPyfft:
+    """The reference's resolution-parameterised transform copies
+    (exoplasim/pyfft.f90), FP64, no filter (nfs=0)."""
+
+    def __init__(self):
+        self.lib = ctypes.CDLL(_ref_path("libref_pyfft.so"))
+
+    def legini(self, ntru, nlat):
+        nlon = 2 * nlat
+        ncsp = (ntru + 1) * (ntru + 2) // 2
+        tab = np.zeros((8, nlat, ncsp))
+        sfilt = np.zeros(ntru + 1)
+        args = [byref(c_int(nlat)), byref(c_int(nlon)), byref(c_int(ntru)),
+                byref(c_int(1))]
+        bufs = [np.zeros((nlat, ncsp)) for _ in range(8)]
+        args += [_p(b) for b in bufs]
+        self.lib.legini_(*args, _p(sfilt), byref(c_int(0)))
+        for i in range(8):
+            tab[i] = bufs[i]
+        return tab
+
+    def sp2fc(self, ntru, nlat, sp):
+        fc = np.zeros((nlat, 2 * nlat))
+        self.lib.sp2fc_(_p(np.ascontiguousarray(sp)), _p(fc), byref(c_int(nlat)),
+                        byref(c_int(2 * nlat)), byref(c_int(ntru)), byref(c_int(1)),
+                        byref(c_int(0)))
+        return fc
+
+    def fc2sp(self, ntru, nlat, fc):
+        ncsp = (ntru + 1) * (ntru + 2) // 2
+        sp = np.zeros(2 * ncsp)
+        self.lib.fc2sp_(_p(np.ascontiguousarray(fc)), _p(sp), byref(c_int(nlat)),
+                        byref(c_int(2 * nlat)), byref(c_int(ntru)), byref(c_int(1)),
+                        byref(c_int(0)))
+        return sp
+
+
+class RefFft991:
+    """The reference's FFT991 module (src/fft991mod.f90), FP64."""
+
+    def __init__(self):
+        self.lib = ctypes.CDLL(_ref_path("libref_fft991.so"))
+
+    def gp2fc(self, a):
+        a = np.array(a, dtype=np.float64, order="C")
+        n = a.shape[-1]
+        self.lib.gp2fc_(_p(a), byref(c_int(n)), byref(c_int(a.size // n)))
+        return a
+
+    def fc2gp(self, a):
+        a = np.array(a, dtype=np.float64, order="C")
+        n = a.shape[-1]
+        self.lib.fc2gp_(_p(a), byref(c_int(n)), byref(c_int(a.size // n)))
+        return a
