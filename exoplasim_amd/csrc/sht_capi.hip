@@ -61,8 +61,6 @@ static void free_tables(Tables& t) {
   for (int i = 0; i < NTAB; ++i) {
     if (t.inv_bf[i]) (void)hipFree(t.inv_bf[i]);
     if (t.inv_f32[i]) (void)hipFree(t.inv_f32[i]);
-    if (t.fwd_bf[i]) (void)hipFree(t.fwd_bf[i]);
-    if (t.fwd_f32[i]) (void)hipFree(t.fwd_f32[i]);
   }
   t = Tables();
 }
@@ -77,8 +75,6 @@ static int build_tables(Tables& t, int ntru, const double* sid, const double* gw
   t.npair = nlat / 2;
   t.ncsp = (ntru + 1) * (ntru + 2) / 2;
   t.kbmax = (ntru + 1 + 31) / 32;
-  t.rbmax = ((ntru + 2) / 2 + 15) / 16;
-  t.kpad = ((t.npair + 31) / 32) * 32;
   t.plavor = plavor;
   const long tsz = (long)t.npair * t.ncsp;
   HIPCK(hipMalloc(&t.sid, sizeof(double) * nlat));
@@ -110,7 +106,6 @@ static int build_tables(Tables& t, int ntru, const double* sid, const double* gw
                                                     t.filt64, t.filt32);
   HIPCK(hipGetLastError());
   t.inv_elems = (long)(t.npair / 8) * (ntru + 1) * t.kbmax * 512;
-  t.fwd_elems = (long)(ntru + 1) * 2 * t.rbmax * 16 * t.kpad;
   const int inv_tabs[] = {TQI, TQJ, TQU, TQV};
   for (int tt : inv_tabs) {
     HIPCK(hipMalloc(&t.inv_bf[tt], sizeof(uint16_t) * 3 * t.inv_elems));
@@ -118,14 +113,6 @@ static int build_tables(Tables& t, int ntru, const double* sid, const double* gw
     k_pack_inv<<<(unsigned)((t.inv_elems + 255) / 256), 256>>>(ntru, t.npair, t.kbmax, tt,
                                                               t.filt64, t.inv_bf[tt],
                                                               t.inv_f32[tt]);
-    HIPCK(hipGetLastError());
-  }
-  const int fwd_tabs[] = {TQC, TQE, TQM, TQQ};
-  for (int tt : fwd_tabs) {
-    HIPCK(hipMalloc(&t.fwd_bf[tt], sizeof(uint16_t) * 3 * t.fwd_elems));
-    HIPCK(hipMalloc(&t.fwd_f32[tt], sizeof(float) * t.fwd_elems));
-    k_pack_fwd<<<(unsigned)((t.fwd_elems + 255) / 256), 256>>>(
-        ntru, t.npair, t.rbmax, t.kpad, tt, t.filt64, t.fwd_bf[tt], t.fwd_f32[tt]);
     HIPCK(hipGetLastError());
   }
   k_twiddle<<<(t.nlon + 255) / 256, 256>>>(t.nlon, t.tw64, t.tw32);

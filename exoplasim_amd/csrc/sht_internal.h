@@ -13,8 +13,6 @@ __global__ void k_filtered(int ntru, int npair, const double* tab, const double*
                            const double* skspgp, double* out64, float* out32);
 __global__ void k_pack_inv(int ntru, int npair, int kbmax, int t, const double* filt64,
                            uint16_t* bf, float* f32);
-__global__ void k_pack_fwd(int ntru, int npair, int rbmax, int kpad, int t,
-                           const double* filt64, uint16_t* bf, float* f32);
 __global__ void k_twiddle(int n, double* tw64, float* tw32);
 
 // Which inverse transform a launch performs.
@@ -26,7 +24,7 @@ enum InvKind : int {
 // Device tables of one truncation (all FP64 values computed on the GPU).
 struct Tables {
   int ntru = 0, nlat = 0, nlon = 0, npair = 0, ncsp = 0;
-  int kbmax = 0, rbmax = 0, kpad = 0;
+  int kbmax = 0;
   double* sid = nullptr;      // [nlat]
   double* gwd = nullptr;      // [nlat]
   double* tab = nullptr;      // [8][npair][ncsp] unfiltered
@@ -36,9 +34,7 @@ struct Tables {
   double* skspgp = nullptr;
   uint16_t* inv_bf[NTAB] = {};  // packed inverse A operands, 3 planes
   float* inv_f32[NTAB] = {};
-  uint16_t* fwd_bf[NTAB] = {};  // packed direct A operands, 3 planes
-  float* fwd_f32[NTAB] = {};
-  long inv_elems = 0, fwd_elems = 0;
+  long inv_elems = 0;
   double* tw64 = nullptr;     // [nlon] complex
   float* tw32 = nullptr;
   double plavor = 0.0;

@@ -5,6 +5,9 @@
 // src/legmod.f90:30-116 (same recurrences and evaluation order, FP64).
 #include "sht_internal.h"
 
+// no FMA contraction: the recurrences round like the reference's
+#pragma clang fp contract(off)
+
 namespace exo {
 
 // ---- Gaussian latitudes: Newton iteration on P_klat (src/gaussmod.f90) ----
@@ -174,35 +177,10 @@ __global__ void k_pack_inv(int ntru, int npair, int kbmax, int t, const double* 
   bf[i] = h;
   bf[total + i] = mm;
   bf[2 * total + i] = l;
-  f32[i] = (float)v;
-}
-
-// ---- direct A operand: [m][cls][rb][row 16][kp], kp = pair (padded to
-// kpad), n = m + par + 2 (16 rb + row), par = parity of the class ----
-__global__ void k_pack_fwd(int ntru, int npair, int rbmax, int kpad, int t,
-                           const double* filt64, uint16_t* bf, float* f32) {
-  const int ncsp = (ntru + 1) * (ntru + 2) / 2;
-  const long tsz = (long)npair * ncsp;
-  const long total = (long)(ntru + 1) * 2 * rbmax * 16 * kpad;
-  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  const int kp = (int)(i % kpad);
-  const int row = (int)((i / kpad) % 16);
-  const int rb = (int)((i / (16L * kpad)) % rbmax);
-  const int cls = (int)((i / (16L * kpad * rbmax)) % 2);
-  const int m = (int)(i / (32L * kpad * rbmax));
-  // sym class: n-m even for P-type, odd for Q-type
-  const int par = (cls == 0) ? (table_is_q(t) ? 1 : 0) : (table_is_q(t) ? 0 : 1);
-  const int n = m + par + 2 * (16 * rb + row);
-  double v = 0.0;
-  if (n <= ntru && kp < npair)
-    v = filt64[t * tsz + (long)kp * ncsp + w0_of(ntru, m) + (n - m)];
-  uint16_t h, mm, l;
-  split3_f64(v, h, mm, l);
-  bf[i] = h;
-  bf[total + i] = mm;
-  bf[2 * total + i] = l;
-  f32[i] = (float)v;
+  // the f32 operand of v_mfma_f32_16x16x4_f32 is stored [k%4][row][k/4] per
+  // 512-element block so that one lane reads its 8 values contiguously
+  const long blk = i - (i % 512);
+  f32[blk + ((k % 4) * 16 + row) * 8 + k / 4] = (float)v;
 }
 
 // twiddles e^{+2 pi i k/N} in FP64 and FP32

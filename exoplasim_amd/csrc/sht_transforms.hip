@@ -39,13 +39,11 @@ struct InvCfg {
 
 template <int T, int MODE>
 struct FwdCfg {
-  static constexpr bool MF = (MODE == MODE_F32MFMA || MODE == MODE_BF16X3);
   static constexpr int PB = (T == 21) ? 16
                           : (T == 42) ? 32
                           : (T == 63) ? (MODE == MODE_F64 ? 24 : 48)
                                       : (MODE == MODE_F64 ? 16 : 32);
   static constexpr int FBF = (T == 21) ? 4 : (T == 42) ? (MODE == MODE_F64 ? 1 : 2) : 1;
-  static constexpr int PBP = MF ? ((PB + 31) / 32) * 32 : PB;
   static constexpr int NPB = TS<T>::NPAIR / PB;
 };
 
@@ -122,62 +120,146 @@ __global__ __launch_bounds__(256) void k_inv(InvArgs a, const void* tabA0, const
 
   // ---- phase 1: Legendre sums into fcl ----
   if constexpr (MODE == MODE_BF16X3 || MODE == MODE_F32MFMA) {
+    // GEMM per m: C[row = 2*pair + class][col] = sum_n A[row][n] B[n][col];
+    // wave wv takes m = wv, wv+4, ...; each (m, 32-wide n block) is one
+    // work item, and the operands of the next item are fetched before the
+    // current one is multiplied.
     const int lane = tid & 63, wv = tid >> 6;
     const int q = lane >> 4, c = lane & 15;
-    for (int m = wv; m <= T; m += 4) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    // column c of B: SCALAR field g*FB + c/2; UV level g*FB/2 + c/4 of
+    // vorticity ((c/2)&1 == 0) or divergence; component c&1
+    const R* bp = nullptr;
+    bool isvor = false;
+    if (KIND == INV_SCALAR) {
+      const int field = g * FB + (c >> 1);
+      if ((c >> 1) < FB && field < a.nfield) bp = spm + (long)(a.src0 + field) * S::NRSP + (c & 1);
+    } else {
+      const int lvl = g * (FB / 2) + (c >> 2);
+      const int s = (c >> 1) & 1;
+      if ((c >> 2) < FB / 2 && lvl < a.nlev) {
+        bp = spm + (long)((s ? a.div0 : a.vor0) + lvl) * S::NRSP + (c & 1);
+        isvor = (s == 0) && ((c & 1) == 0);
+      }
+    }
+    const R pv = (R)a.plavor;
+    // n offset of B element j (bf16: 8 consecutive n per lane; f32: n = 4j + q)
+    auto noff = [&](int j) { return MODE == MODE_BF16X3 ? q * 8 + j : j * 4 + q; };
+    auto fetch = [&](int m, int kb, float* br, uint4* a0, uint4* a1) {
       const int w0 = w0_of(T, m);
-      const int kbn = (NM - m + 31) / 32;
-      for (int kb = 0; kb < kbn; ++kb) {
 #pragma unroll
-        for (int term = 0; term < NTERM; ++term) {
-          const long abase = (((long)(pb * NM + m) * S::KB + kb) * 16 + c) * 32;
-          if constexpr (MODE == MODE_BF16X3) {
-            const uint16_t* A = (const uint16_t*)(term == 0 ? tabA0 : tabA1);
-            BF8 ah, am, al, bh, bm, bl;
-            ah.q = *(const uint4*)(A + abase + q * 8);
-            am.q = *(const uint4*)(A + plane + abase + q * 8);
-            al.q = *(const uint4*)(A + 2 * plane + abase + q * 8);
+      for (int j = 0; j < 8; ++j) {
+        const int n = m + kb * 32 + noff(j);
+        R x = R(0);
+        if (bp && n <= T) {
+          x = bp[2 * (w0 + n - m)];
+          if (isvor && m == 0 && n == 1) x = x - pv;
+        }
+        br[j] = (float)x;
+      }
+      const long blk = ((long)(pb * NM + m) * S::KB + kb) * 512;
+      if constexpr (MODE == MODE_BF16X3) {
+        const uint16_t* A0 = (const uint16_t*)tabA0 + blk + c * 32 + q * 8;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-              const int n = m + kb * 32 + q * 8 + j;
-              const float x = (n <= T) ? (float)bval(term, c, w0 + n - m) : 0.f;
-              split3(x, bh.s[j], bm.s[j], bl.s[j]);
-            }
-            acc = mfma_bf16(ah, bl, acc);
-            acc = mfma_bf16(am, bm, acc);
-            acc = mfma_bf16(al, bh, acc);
-            acc = mfma_bf16(ah, bm, acc);
-            acc = mfma_bf16(am, bh, acc);
-            acc = mfma_bf16(ah, bh, acc);
-          } else {
-            const float* A = (const float*)(term == 0 ? tabA0 : tabA1);
+        for (int p = 0; p < 3; ++p) a0[p] = *(const uint4*)(A0 + p * plane);
+        if (NTERM == 2) {
+          const uint16_t* A1 = (const uint16_t*)tabA1 + blk + c * 32 + q * 8;
 #pragma unroll
-            for (int ks = 0; ks < 8; ++ks) {
-              const int k = ks * 4 + q;
-              const int n = m + kb * 32 + k;
-              const float x = (n <= T) ? (float)bval(term, c, w0 + n - m) : 0.f;
-              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A[abase + k], x, acc, 0, 0, 0);
-            }
-          }
+          for (int p = 0; p < 3; ++p) a1[p] = *(const uint4*)(A1 + p * plane);
+        }
+      } else {
+        // f32 operand stored as [q][row][8] so one lane reads 8 floats
+        const float* A0 = (const float*)tabA0 + blk + (q * 16 + c) * 8;
+        a0[0] = *(const uint4*)A0;
+        a0[1] = *(const uint4*)(A0 + 4);
+        if (NTERM == 2) {
+          const float* A1 = (const float*)tabA1 + blk + (q * 16 + c) * 8;
+          a1[0] = *(const uint4*)A1;
+          a1[1] = *(const uint4*)(A1 + 4);
         }
       }
-      // rows 4q..4q+3 = (pair 2q, sym), (2q, anti), (2q+1, sym), (2q+1, anti)
-      const int fl = c >> 1, ri = c & 1;
-      if (fl < FB) {
+    };
+    auto mul = [&](const float* br, const uint4* a, f32x4 acc) -> f32x4 {
+      if constexpr (MODE == MODE_BF16X3) {
+        BF8 bh, bm, bl, ah, am, al;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int pl = 2 * q + h;
-          const float sy = acc[2 * h], an = acc[2 * h + 1];
-          NS2<R>& e = fcl[(pl * FB + fl) * NM + m];
-          if (ri == 0) {
-            e.n.x = sy + an;
-            e.s.x = sy - an;
-          } else {
-            e.n.y = sy + an;
-            e.s.y = sy - an;
+        for (int j = 0; j < 8; ++j) split3(br[j], bh.s[j], bm.s[j], bl.s[j]);
+        ah.q = a[0];
+        am.q = a[1];
+        al.q = a[2];
+        acc = mfma_bf16(ah, bl, acc);
+        acc = mfma_bf16(am, bm, acc);
+        acc = mfma_bf16(al, bh, acc);
+        acc = mfma_bf16(ah, bm, acc);
+        acc = mfma_bf16(am, bh, acc);
+        acc = mfma_bf16(ah, bh, acc);
+      } else {
+        const float* af = (const float*)a;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[j], br[j], acc, 0, 0, 0);
+      }
+      return acc;
+    };
+    float br[8], br2[8];
+    uint4 a0[3], a1[3], a02[3], a12[3];
+    int m = wv, kb = 0;
+    if (m <= T) fetch(m, 0, br, a0, a1);
+    f32x4 acc1 = {0.f, 0.f, 0.f, 0.f}, acc2 = {0.f, 0.f, 0.f, 0.f};
+    while (m <= T) {
+      const int kbn = (NM - m + 31) / 32;
+      int m2 = m, kb2 = kb + 1;
+      if (kb2 >= kbn) {
+        m2 = m + 4;
+        kb2 = 0;
+      }
+      if (m2 <= T) fetch(m2, kb2, br2, a02, a12);
+      acc1 = mul(br, a0, acc1);
+      if (NTERM == 2) acc2 = mul(br, a1, acc2);
+      if (kb2 == 0) {
+        // all n of this m are summed: combine the terms, fold N/S, store
+        f32x4 out = acc1;
+        if (KIND == INV_UV) {
+          // u = qv.zeta + qu.(-i D), v = qu.(-i zeta) - qv.D: the qu product of
+          // the partner column (zeta.re <-> D.im, zeta.im <-> D.re)
+          const int plane_ = (lane & ~3) | (3 - (lane & 3));
+          f32x4 p2;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) p2[i] = __shfl(acc2[i], plane_, 64);
+          const int r4 = c & 3;
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            out[i] = (r4 == 0) ? acc1[i] + p2[i]
+                   : (r4 == 1) ? acc1[i] - p2[i]
+                   : (r4 == 2) ? p2[i] - acc1[i]
+                               : -p2[i] - acc1[i];
+        }
+        const int fl = c >> 1, ri = c & 1;
+        if (fl < FB) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int pl = 2 * q + h;
+            const float sy = out[2 * h], an = out[2 * h + 1];
+            NS2<R>& e = fcl[(pl * FB + fl) * NM + m];
+            if (ri == 0) {
+              e.n.x = sy + an;
+              e.s.x = sy - an;
+            } else {
+              e.n.y = sy + an;
+              e.s.y = sy - an;
+            }
           }
         }
+        acc1 = f32x4{0.f, 0.f, 0.f, 0.f};
+        acc2 = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+      m = m2;
+      kb = kb2;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) br[j] = br2[j];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        a0[p] = a02[p];
+        a1[p] = a12[p];
       }
     }
   } else {
@@ -286,19 +368,25 @@ __global__ __launch_bounds__(256) void k_inv(InvArgs a, const void* tabA0, const
 // ---------------------------------------------------------------------------
 // direct transform
 // ---------------------------------------------------------------------------
+// One block transforms FBF field-levels of one member over PB latitude pairs
+// (all pairs when NPB == 1, else partial sums reduced by k_reduce_partials).
+// The direct Legendre sum runs on the vector ALUs in every mode: a block can
+// hold the Fourier coefficients of only FBF (1-4) fields in LDS, which would
+// fill 2-8 of the 16 columns of an MFMA tile while the operand split would
+// still cost the whole wave (see the design notes).
 template <int T, int MODE, bool FUSED>
-__global__ __launch_bounds__(256) void k_fwd(FwdArgs a, const void* tabA, long plane,
-                                             const void* tabv, const void* twp, void* scratch) {
+__global__ __launch_bounds__(256) void k_fwd(FwdArgs a, const void* tabv, const void* twp,
+                                             void* scratch) {
   using S = TS<T>;
   using R = RealT<MODE>;
   using C = FwdCfg<T, MODE>;
   constexpr int NM = S::NM, N = S::NLON, NLAT = S::NLAT;
-  constexpr int PB = C::PB, FBF = C::FBF, PBP = C::PBP, NPB = C::NPB;
+  constexpr int PB = C::PB, FBF = C::FBF, NPB = C::NPB;
   constexpr int NCOL = 2 * FBF;
   constexpr int NF = FftBatch<N>::NF;
 
-  // F[m][cls][col][pair]: E (cls 0) = fc_N + fc_S, O (cls 1) = fc_N - fc_S
-  __shared__ R F[NM * 2 * NCOL * PBP];
+  // F[m][cls][pair][col]: cls 0 = fc_N + fc_S (E), cls 1 = fc_N - fc_S (O)
+  __shared__ R F[NM * 2 * PB * NCOL];
   __shared__ C2<R> fbuf[FUSED ? NF * N : 1];
   __shared__ C2<R> tws[FUSED ? N : 1];
   __shared__ uint8_t mw[S::NCSP];
@@ -311,7 +399,6 @@ __global__ __launch_bounds__(256) void k_fwd(FwdArgs a, const void* tabA, long p
   if (mem >= a.nmem) return;
   const R* inm = (const R*)a.in + mem * a.in_stride;
 
-  for (int i = tid; i < NM * 2 * NCOL * PBP; i += 256) F[i] = R(0);
   for (int m = tid; m <= T; m += 256)
     for (int n = m; n <= T; ++n) mw[w0_of(T, m) + n - m] = (uint8_t)m;
   if (FUSED) {
@@ -320,7 +407,7 @@ __global__ __launch_bounds__(256) void k_fwd(FwdArgs a, const void* tabA, long p
   }
   __syncthreads();
 
-  auto fidx = [&](int m, int cls, int col, int pl) { return ((m * 2 + cls) * NCOL + col) * PBP + pl; };
+  auto fidx = [](int m, int cls, int pl, int col) { return ((m * 2 + cls) * PB + pl) * NCOL + col; };
 
   if constexpr (FUSED) {
     for (int f0 = 0; f0 < PB * FBF; f0 += NF) {
@@ -338,22 +425,22 @@ __global__ __launch_bounds__(256) void k_fwd(FwdArgs a, const void* tabA, long p
       // separate the two real rows, scale by 1/N, fold into E/O
       const R sc = R(0.5) / R(N);
       for (int idx = tid; idx < nf * NM; idx += 256) {
-        const int f = idx / NM, k = idx % NM;
+        const int f = idx % nf, k = idx / nf;
         const C2<R> zk = fbuf[f * N + k];
         const C2<R> zn = fbuf[f * N + ((N - k) % N)];
         const R cnx = (zk.x + zn.x) * sc, cny = (zk.y - zn.y) * sc;
         const R csx = (zk.y + zn.y) * sc, csy = (zn.x - zk.x) * sc;
         const int gi = f0 + f, pl = gi / FBF, fl = gi % FBF;
-        F[fidx(k, 0, 2 * fl, pl)] = cnx + csx;
-        F[fidx(k, 0, 2 * fl + 1, pl)] = cny + csy;
-        F[fidx(k, 1, 2 * fl, pl)] = cnx - csx;
-        F[fidx(k, 1, 2 * fl + 1, pl)] = cny - csy;
+        F[fidx(k, 0, pl, 2 * fl)] = cnx + csx;
+        F[fidx(k, 0, pl, 2 * fl + 1)] = cny + csy;
+        F[fidx(k, 1, pl, 2 * fl)] = cnx - csx;
+        F[fidx(k, 1, pl, 2 * fl + 1)] = cny - csy;
       }
       __syncthreads();
     }
   } else {
     for (int idx = tid; idx < PB * FBF * NM; idx += 256) {
-      const int k = idx % NM, pf = idx / NM;
+      const int pf = idx % (PB * FBF), k = idx / (PB * FBF);
       const int fl = pf % FBF, pl = pf / FBF;
       const int field = g * FBF + fl;
       if (field >= a.nfield) continue;
@@ -362,99 +449,39 @@ __global__ __launch_bounds__(256) void k_fwd(FwdArgs a, const void* tabA, long p
       const R nx = src[(long)lat * N + 2 * k], ny = src[(long)lat * N + 2 * k + 1];
       const R sx = src[(long)(NLAT - 1 - lat) * N + 2 * k];
       const R sy = src[(long)(NLAT - 1 - lat) * N + 2 * k + 1];
-      F[fidx(k, 0, 2 * fl, pl)] = nx + sx;
-      F[fidx(k, 0, 2 * fl + 1, pl)] = ny + sy;
-      F[fidx(k, 1, 2 * fl, pl)] = nx - sx;
-      F[fidx(k, 1, 2 * fl + 1, pl)] = ny - sy;
+      F[fidx(k, 0, pl, 2 * fl)] = nx + sx;
+      F[fidx(k, 0, pl, 2 * fl + 1)] = ny + sy;
+      F[fidx(k, 1, pl, 2 * fl)] = nx - sx;
+      F[fidx(k, 1, pl, 2 * fl + 1)] = ny - sy;
     }
     __syncthreads();
   }
 
-  // output location of (field-local fl, spectral w, ri)
   R* outp;
-  long ostride;
-  if (NPB == 1) {
+  if (NPB == 1)
     outp = (R*)a.sp + mem * a.sp_stride + (long)a.dst0 * S::NRSP;
-    ostride = S::NRSP;
-  } else {
+  else
     outp = (R*)scratch + ((long)pb * a.nmem + mem) * a.nfield * S::NRSP;
-    ostride = S::NRSP;
-  }
   const bool qtab = table_is_q(a.table);
-
-  if constexpr (C::MF) {
-    const int lane = tid & 63, wv = tid >> 6;
-    const int q = lane >> 4, c = lane & 15;
-    constexpr int RB = S::RB;
-    const long kpad = ((S::NPAIR + 31) / 32) * 32;
-    for (int task = wv; task < NM * 2 * RB; task += 4) {
-      const int m = task / (2 * RB), cls = (task / RB) % 2, rb = task % RB;
-      const int par = (cls == 0) ? (qtab ? 1 : 0) : (qtab ? 0 : 1);
-      if (m + par + 32 * rb > T) continue;
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      const long arow = ((long)((m * 2 + cls) * RB + rb) * 16 + c) * kpad + pb * PB;
-      for (int kk = 0; kk < PBP; kk += 32) {
-        if constexpr (MODE == MODE_BF16X3) {
-          const uint16_t* A = (const uint16_t*)tabA;
-          BF8 ah, am, al, bh, bm, bl;
-          ah.q = *(const uint4*)(A + arow + kk + q * 8);
-          am.q = *(const uint4*)(A + plane + arow + kk + q * 8);
-          al.q = *(const uint4*)(A + 2 * plane + arow + kk + q * 8);
+  const R* tv = (const R*)tabv + (long)a.table * S::NPAIR * S::NCSP + (long)pb * PB * S::NCSP;
+  for (int w = tid; w < S::NCSP; w += 256) {
+    const int m = mw[w];
+    const int n = m + (w - w0_of(T, m));
+    const int cls = ((((n - m) & 1) == 0) != qtab) ? 0 : 1;
+    const R* fr = &F[fidx(m, cls, 0, 0)];
+    R acc[NCOL];
 #pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            const float x = (c < NCOL) ? (float)F[fidx(m, cls, c, kk + q * 8 + j)] : 0.f;
-            split3(x, bh.s[j], bm.s[j], bl.s[j]);
-          }
-          acc = mfma_bf16(ah, bl, acc);
-          acc = mfma_bf16(am, bm, acc);
-          acc = mfma_bf16(al, bh, acc);
-          acc = mfma_bf16(ah, bm, acc);
-          acc = mfma_bf16(am, bh, acc);
-          acc = mfma_bf16(ah, bh, acc);
-        } else {
-          const float* A = (const float*)tabA;
+    for (int c = 0; c < NCOL; ++c) acc[c] = R(0);
+#pragma unroll 4
+    for (int pl = 0; pl < PB; ++pl) {
+      const R p = tv[(long)pl * S::NCSP + w];
 #pragma unroll
-          for (int ks = 0; ks < 8; ++ks) {
-            const int k = kk + ks * 4 + q;
-            const float x = (c < NCOL) ? (float)F[fidx(m, cls, c, k)] : 0.f;
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A[arow + k], x, acc, 0, 0, 0);
-          }
-        }
-      }
-      if (c < NCOL) {
-        const int fl = c >> 1, ri = c & 1;
-        const int field = g * FBF + fl;
-        if (field < a.nfield) {
-          R* o = outp + (long)field * ostride;
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int n = m + par + 2 * (16 * rb + 4 * q + i);
-            if (n <= T) o[2 * (w0_of(T, m) + n - m) + ri] = (R)acc[i];
-          }
-        }
-      }
+      for (int c = 0; c < NCOL; ++c) acc[c] += p * fr[pl * NCOL + c];
     }
-  } else {
-    const R* tv = (const R*)tabv;
-    const long tsz = (long)S::NPAIR * S::NCSP;
-    for (int w = tid; w < S::NCSP; w += 256) {
-      const int m = mw[w];
-      const int n = m + (w - w0_of(T, m));
-      const bool even = ((n - m) & 1) == 0;
-      const int cls = (even != qtab) ? 0 : 1;
-      R acc[NCOL];
 #pragma unroll
-      for (int c = 0; c < NCOL; ++c) acc[c] = R(0);
-      for (int pl = 0; pl < PB; ++pl) {
-        const R p = tv[a.table * tsz + (long)(pb * PB + pl) * S::NCSP + w];
-#pragma unroll
-        for (int c = 0; c < NCOL; ++c) acc[c] += p * F[fidx(m, cls, c, pl)];
-      }
-#pragma unroll
-      for (int c = 0; c < NCOL; ++c) {
-        const int field = g * FBF + (c >> 1);
-        if (field < a.nfield) outp[(long)field * ostride + 2 * w + (c & 1)] = acc[c];
-      }
+    for (int c = 0; c < NCOL; ++c) {
+      const int field = g * FBF + (c >> 1);
+      if (field < a.nfield) outp[(long)field * S::NRSP + 2 * w + (c & 1)] = acc[c];
     }
   }
 }
@@ -597,16 +624,13 @@ static int fwd_t(const Tables& tb, bool fused, const FwdArgs& a, void* scratch, 
   if (nblk <= 0) return 0;
   if (C::NPB > 1 && (scratch == nullptr || sbytes < fwd_scratch_t<T, MODE>(a.nmem, a.nfield)))
     return -4;
-  const void* A = nullptr;
-  if (MODE == MODE_BF16X3) A = tb.fwd_bf[a.table];
-  if (MODE == MODE_F32MFMA) A = tb.fwd_f32[a.table];
   const void* tv = (MODE == MODE_F64) ? (const void*)tb.filt64 : (const void*)tb.filt32;
   const void* tw = (MODE == MODE_F64) ? (const void*)tb.tw64 : (const void*)tb.tw32;
   dim3 grid((unsigned)nblk), block(256);
   if (fused)
-    k_fwd<T, MODE, true><<<grid, block, 0, s>>>(a, A, tb.fwd_elems, tv, tw, scratch);
+    k_fwd<T, MODE, true><<<grid, block, 0, s>>>(a, tv, tw, scratch);
   else
-    k_fwd<T, MODE, false><<<grid, block, 0, s>>>(a, A, tb.fwd_elems, tv, tw, scratch);
+    k_fwd<T, MODE, false><<<grid, block, 0, s>>>(a, tv, tw, scratch);
   if (C::NPB > 1) {
     const long total = (long)a.nmem * a.nfield * TS<T>::NRSP;
     k_reduce_partials<R><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(

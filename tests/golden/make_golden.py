@@ -1,0 +1,86 @@
+"""Write the golden vectors of the spectral-transform path from the
+reference's own Fortran, compiled by oracle/Makefile into oracle/_ref/.
+
+Run in a tree where oracle/_ref/ is built (needs the reference sources):
+    make -C oracle ref && python tests/golden/make_golden.py
+It writes tests/golden/*.npz: inputs are closed-form deterministic
+functions (golden_input) so only outputs are stored; arrays are subsets
+where the full output would be large.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+
+def golden_input(shape, salt: float) -> np.ndarray:
+    """Deterministic test input without a random generator:
+    x_i = sin(0.7 i + salt) * cos(0.013 i^1.5 + 2 salt)."""
+    i = np.arange(int(np.prod(shape)), dtype=np.float64)
+    return (np.sin(0.7 * i + salt) * np.cos(0.013 * i ** 1.5 + 2.0 * salt)).reshape(shape)
+
+
+def spectral_input(ntru, nfl, salt):
+    """golden_input shaped as spectral field-levels, Im(m=0) zeroed and
+    damped with (n+1)^-1.5."""
+    m = np.concatenate([np.full(ntru + 1 - mm, mm) for mm in range(ntru + 1)])
+    n = np.concatenate([np.arange(mm, ntru + 1) for mm in range(ntru + 1)])
+    x = golden_input((nfl, m.size, 2), salt)
+    x *= ((n + 1.0) ** -1.5)[None, :, None]
+    x[:, m == 0, 1] = 0.0
+    return x.reshape(nfl, 2 * m.size)
+
+
+LAT_SUBSET = [0, 5, 15, 16, 31]  # rows of the T21 tables kept (N and S)
+
+
+def main():
+    from oracle import RefFft991, RefPyfft, RefT21
+
+    r = RefT21()
+    out = {}
+    out["sid"], out["gwd"] = r.sid, r.gwd
+    out["tables"] = r.tables[:, LAT_SUBSET, :]
+    out["plavor"] = np.array([r.plavor])
+    nl = r.nlev
+    sp = spectral_input(21, 3 * nl + 1, 0.3)
+    out["sp2fc_f0"] = r.sp2fc(sp[0])
+    out["sp2fcdmu_lnps"] = r.sp2fcdmu(sp[-1])
+    out["fc2sp_f0"] = r.fc2sp(out["sp2fc_f0"])
+    pu, pv = r.dv2uv(sp[nl:2 * nl], sp[:nl])
+    out["dv2uv_u01"], out["dv2uv_v01"] = pu[:2], pv[:2]
+    f = [golden_input((nl, r.nlat, r.nlon), 1.1 + 0.5 * k) for k in range(6)]
+    pd, pz = r.uv2dv(f[0], f[1])
+    out["uv2dv_d01"], out["uv2dv_z01"] = pd[:2], pz[:2]
+    out["qtend_01"] = r.qtend(f[0], f[1], f[2])[:2]
+    d, t, z = r.mktend(*f)
+    out["mktend_d01"], out["mktend_t01"], out["mktend_z01"] = d[:2], t[:2], z[:2]
+    for n in (64, 128, 384):
+        g = golden_input((3, n), 0.1 * n)
+        out[f"gp2fc_{n}"] = r.gp2fc(g)
+        out[f"fc2gp_{n}"] = r.fc2gp(g)
+    f9 = RefFft991()
+    for n in (192, 128):
+        g = golden_input((3, n), 0.2 * n)
+        out[f"fft991_gp2fc_{n}"] = f9.gp2fc(g)
+        out[f"fft991_fc2gp_{n}"] = f9.fc2gp(g)
+    p = RefPyfft()
+    tab42 = p.legini(42, 64)
+    out["t42_tables_lat0_7"] = tab42[:, [0, 7], :]
+    sp42 = spectral_input(42, 1, 0.9)[0]
+    fc42 = p.sp2fc(42, 64, sp42)
+    out["t42_sp2fc"] = fc42
+    out["t42_fc2sp"] = p.fc2sp(42, 64, fc42)
+    path = os.path.join(HERE, "reference_t21_t42.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes")
+
+
+if __name__ == "__main__":
+    main()

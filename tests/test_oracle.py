@@ -1,0 +1,287 @@
+"""Pins of the oracle (oracle/exo_oracle.c) -- CPU only.
+
+Each function of the oracle is checked against (a) the golden vectors written
+from the reference's own compiled Fortran (tests/golden/make_golden.py), (b)
+the compiled reference itself when oracle/_ref/ is built in this tree, and
+(c) what the mathematics fixes independently of the reference: numpy's
+Gauss-Legendre nodes, scipy's associated Legendre functions, numpy's real
+FFT, quadrature orthonormality, the round-trip identity, closed-form wind
+fields and numpy's matrix inverse.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+from make_golden import LAT_SUBSET, golden_input, spectral_input  # noqa: E402
+
+GOLD = os.path.join(HERE, "golden", "reference_t21_t42.npz")
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(GOLD)
+
+
+def close(a, b, rtol):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    scale = max(np.abs(b).max(), 1e-300)
+    return np.abs(a - b).max() / scale < rtol
+
+
+# ---------------------------------------------------------------------------
+# (a) golden vectors of the compiled reference
+# ---------------------------------------------------------------------------
+def test_golden_grid_tables(oracle, gold):
+    sid, gwd = oracle.inigau(32)
+    assert close(sid, gold["sid"], 1e-15) and close(gwd, gold["gwd"], 1e-15)
+    t = oracle.legini(21, 32, sid, gwd)["_all"]
+    assert close(t[:, LAT_SUBSET, :], gold["tables"], 1e-15)
+    t42 = oracle.legini(42, 64)["_all"]
+    assert close(t42[:, [0, 7], :], gold["t42_tables_lat0_7"], 1e-15)
+
+
+def test_golden_legendre(oracle, gold):
+    nl = 10
+    tab = oracle.legini(21, 32)
+    one = np.ones(22)
+    sp = spectral_input(21, 3 * nl + 1, 0.3)
+    fc = oracle.sp2fc(21, 32, 64, tab["qi"], one, sp[0])
+    assert close(fc, gold["sp2fc_f0"], 1e-15)
+    assert close(oracle.sp2fc(21, 32, 64, tab["qj"], one, sp[-1]), gold["sp2fcdmu_lnps"], 1e-15)
+    assert close(oracle.fc2sp(21, 32, 64, tab["qc"], one, fc), gold["fc2sp_f0"], 1e-15)
+    pu, pv = oracle.dv2uv(21, 32, 64, nl, tab["qu"], tab["qv"], one, float(gold["plavor"][0]),
+                          sp[nl:2 * nl], sp[:nl])
+    assert close(pu[:2], gold["dv2uv_u01"], 1e-15) and close(pv[:2], gold["dv2uv_v01"], 1e-15)
+    f = [golden_input((nl, 32, 64), 1.1 + 0.5 * k) for k in range(6)]
+    pd, pz = oracle.uv2dv(21, 32, 64, nl, tab["qe"], tab["qm"], one, f[0], f[1])
+    assert close(pd[:2], gold["uv2dv_d01"], 1e-14) and close(pz[:2], gold["uv2dv_z01"], 1e-14)
+    q = oracle.qtend(21, 32, 64, nl, tab["qc"], tab["qe"], tab["qm"], one, f[0], f[1], f[2])
+    assert close(q[:2], gold["qtend_01"], 1e-14)
+    d, t, z = oracle.mktend(21, 32, 64, nl, tab["qc"], tab["qe"], tab["qm"], tab["qq"], one, *f)
+    assert close(d[:2], gold["mktend_d01"], 1e-14)
+    assert close(t[:2], gold["mktend_t01"], 1e-14)
+    assert close(z[:2], gold["mktend_z01"], 1e-14)
+    tab42 = oracle.legini(42, 64)
+    sp42 = spectral_input(42, 1, 0.9)[0]
+    fc42 = oracle.sp2fc(42, 64, 128, tab42["qi"], np.ones(43), sp42)
+    assert close(fc42, gold["t42_sp2fc"], 1e-15)
+    assert close(oracle.fc2sp(42, 64, 128, tab42["qc"], np.ones(43), fc42), gold["t42_fc2sp"], 1e-15)
+
+
+@pytest.mark.parametrize("n", [64, 128, 384])
+def test_golden_fftmod(oracle, gold, n):
+    g = golden_input((3, n), 0.1 * n)
+    assert close(oracle.gp2fc(g), gold[f"gp2fc_{n}"], 1e-15)
+    assert close(oracle.fc2gp(g), gold[f"fc2gp_{n}"], 1e-15)
+
+
+@pytest.mark.parametrize("n", [192, 128])
+def test_golden_fft991(oracle, gold, n):
+    g = golden_input((3, n), 0.2 * n)
+    assert close(oracle.gp2fc(g, dft=True), gold[f"fft991_gp2fc_{n}"], 1e-14)
+    assert close(oracle.fc2gp(g, dft=True), gold[f"fft991_fc2gp_{n}"], 1e-14)
+
+
+# ---------------------------------------------------------------------------
+# (b) the compiled reference, when it is built in this tree
+# ---------------------------------------------------------------------------
+def _ref():
+    try:
+        from oracle import RefT21, ref_available
+    except ImportError:
+        return None
+    return RefT21() if ref_available() else None
+
+
+def test_live_reference_t21(oracle):
+    r = _ref()
+    if r is None:
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    t = oracle.legini(21, 32, r.sid, r.gwd)["_all"]
+    assert np.array_equal(t, r.tables)
+    rng = np.random.default_rng(11)
+    one = np.ones(22)
+    sp = rng.standard_normal(2 * r.ncsp)
+    fc = oracle.sp2fc(21, 32, 64, t[0], one, sp)
+    assert np.array_equal(fc, r.sp2fc(sp))
+    assert np.array_equal(oracle.fc2sp(21, 32, 64, t[2], one, fc), r.fc2sp(fc))
+    g = rng.standard_normal((5, 64))
+    assert np.array_equal(oracle.gp2fc(g), r.gp2fc(g))
+    assert np.array_equal(oracle.fc2gp(g), r.fc2gp(g))
+
+
+# ---------------------------------------------------------------------------
+# (c) closed forms and invariants
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("nlat", [32, 64, 96, 192])
+def test_inigau_vs_numpy(oracle, nlat):
+    sid, gwd = oracle.inigau(nlat)
+    x, w = np.polynomial.legendre.leggauss(nlat)
+    assert np.abs(sid - x[::-1]).max() < 1e-14
+    assert np.abs(gwd - w[::-1]).max() < 1e-14
+    assert abs(gwd.sum() - 2.0) < 1e-13
+
+
+def _pbar(m, n, mu):
+    """fully normalised P_n^m with Condon-Shortley phase (int P^2 dmu = 1)"""
+    from scipy.special import gammaln, lpmv
+
+    f = np.exp(0.5 * (np.log((2 * n + 1) / 2.0) + gammaln(n - m + 1) - gammaln(n + m + 1)))
+    return f * lpmv(m, n, mu)
+
+
+def test_legini_vs_scipy(oracle):
+    ntru, nlat = 21, 32
+    sid, gwd = oracle.inigau(nlat)
+    tab = oracle.legini(ntru, nlat, sid, gwd)
+    w = 0
+    for m in range(ntru + 1):
+        for n in range(m, ntru + 1):
+            p = _pbar(m, n, sid)
+            assert np.abs(tab["qi"][:, w] - p).max() < 1e-12, (m, n)
+            # Q = (1 - mu^2) dP/dmu by central differences
+            h = 1e-6
+            dq = (1 - sid ** 2) * (_pbar(m, n, sid + h) - _pbar(m, n, sid - h)) / (2 * h)
+            assert np.abs(tab["qj"][:, w] - dq).max() < 1e-6 * max(1.0, np.abs(dq).max()), (m, n)
+            w += 1
+
+
+@pytest.mark.parametrize("ntru,nlat", [(21, 32), (42, 64), (127, 192)])
+def test_legini_orthonormal(oracle, ntru, nlat):
+    sid, gwd = oracle.inigau(nlat)
+    tab = oracle.legini(ntru, nlat, sid, gwd)
+    qi = tab["qi"]
+    w0 = 0
+    for m in range(ntru + 1):
+        k = ntru + 1 - m
+        p = qi[:, w0:w0 + k]
+        gram = p.T @ (gwd[:, None] * p)
+        # the three-term recurrence accumulates ~ntru ulps
+        assert np.abs(gram - np.eye(k)).max() < 3e-14 * ntru, m
+        # qc is P times the weight
+        assert np.allclose(tab["qc"][:, w0:w0 + k], p * gwd[:, None], rtol=0, atol=1e-16)
+        w0 += k
+
+
+def test_sp2fc_single_mode(oracle):
+    ntru, nlat, nlon = 21, 32, 64
+    sid, gwd = oracle.inigau(nlat)
+    tab = oracle.legini(ntru, nlat, sid, gwd)
+    one = np.ones(ntru + 1)
+    for (m, n) in [(0, 0), (0, 5), (3, 4), (7, 20), (21, 21)]:
+        w = m * (ntru + 1) - m * (m - 1) // 2 + (n - m)
+        sp = np.zeros(2 * tab["qi"].shape[1])
+        sp[2 * w] = 1.0
+        sp[2 * w + 1] = 0.5 if m > 0 else 0.0
+        fc = oracle.sp2fc(ntru, nlat, nlon, tab["qi"], one, sp)
+        p = _pbar(m, n, sid)
+        assert np.abs(fc[:, 2 * m] - p).max() < 1e-12
+        assert np.abs(fc[:, 2 * m + 1] - sp[2 * w + 1] * p).max() < 1e-12
+        other = np.delete(fc.reshape(nlat, nlon // 2, 2), m, axis=1)
+        assert np.abs(other).max() == 0.0
+
+
+@pytest.mark.parametrize("ntru,nlat", [(21, 32), (42, 64), (63, 96)])
+def test_roundtrip_identity(oracle, ntru, nlat):
+    nlev = 2
+    sp = spectral_input(ntru, 3 * nlev + 1, 0.45)
+    gp, sp_out = oracle.roundtrip(ntru, nlev, sp)
+    assert np.abs(sp_out - sp).max() < 1e-13
+
+
+@pytest.mark.parametrize("n", [64, 128, 192, 384])
+def test_fft_vs_numpy(oracle, n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((4, n))
+    X = np.fft.rfft(x, axis=-1) / n
+    ref = np.zeros((4, n))
+    ref[:, 0::2] = X.real[:, : n // 2]
+    ref[:, 1::2] = X.imag[:, : n // 2]
+    ref[:, 1] = 0.0
+    dft = n == 192
+    assert np.abs(oracle.gp2fc(x, dft=dft) - ref).max() < 1e-15
+    # inverse of a spectrum without Im(0) and Nyquist: N * irfft
+    c = ref.copy()
+    full = np.zeros((4, n // 2 + 1), dtype=complex)
+    full[:, : n // 2] = c[:, 0::2] + 1j * c[:, 1::2]
+    full[:, 0] = full[:, 0].real
+    xr = np.fft.irfft(full, n=n, axis=-1) * n
+    assert np.abs(oracle.fc2gp(c, dft=dft) - xr).max() < 1e-13
+
+
+def test_fft_unsupported(oracle):
+    with pytest.raises(ValueError):
+        oracle.gp2fc(np.zeros((1, 100)))
+
+
+def test_dv2uv_closed_forms(oracle):
+    ntru, nlat, nlon, nlev = 21, 32, 64, 2
+    sid, gwd = oracle.inigau(nlat)
+    tab = oracle.legini(ntru, nlat, sid, gwd)
+    one = np.ones(ntru + 1)
+    ncsp = tab["qi"].shape[1]
+    plavor = 1.63299310207
+    pd = np.zeros((nlev, 2 * ncsp))
+    pz = np.zeros((nlev, 2 * ncsp))
+    # pure planetary vorticity: no relative motion
+    pz[:, 2] = plavor
+    pu, pv = oracle.dv2uv(ntru, nlat, nlon, nlev, tab["qu"], tab["qv"], one, plavor, pd, pz)
+    assert np.abs(pu).max() == 0.0 and np.abs(pv).max() == 0.0
+    # solid-body rotation on top: zeta = c P_1^0, U = c sqrt(3/2)(1 - mu^2)/2
+    c = 0.37
+    pz[:, 2] = plavor + c
+    pu, pv = oracle.dv2uv(ntru, nlat, nlon, nlev, tab["qu"], tab["qv"], one, plavor, pd, pz)
+    ucl = c * np.sqrt(1.5) * (1 - sid ** 2) / 2
+    assert np.abs(pu[:, :, 0] - ucl[None, :]).max() < 1e-15
+    assert np.abs(pu[:, :, 1:]).max() < 1e-15 and np.abs(pv).max() < 1e-15
+    # pure divergence of the same shape gives the meridional component
+    pz[:, 2] = plavor
+    pd[:, 2] = c
+    pu, pv = oracle.dv2uv(ntru, nlat, nlon, nlev, tab["qu"], tab["qv"], one, plavor, pd, pz)
+    assert np.abs(pv[:, :, 0] + ucl[None, :]).max() < 1e-15
+    assert np.abs(pu).max() < 1e-15
+
+
+def test_mktend_linear_and_uv2dv_adjoint(oracle):
+    ntru, nlat, nlon, nl = 21, 32, 64, 2
+    tab = oracle.legini(ntru, nlat)
+    one = np.ones(ntru + 1)
+    rng = np.random.default_rng(5)
+    f = [rng.standard_normal((nl, nlat, nlon)) for _ in range(6)]
+    g = [rng.standard_normal((nl, nlat, nlon)) for _ in range(6)]
+    args = (ntru, nlat, nlon, nl, tab["qc"], tab["qe"], tab["qm"], tab["qq"], one)
+    a = oracle.mktend(*args, *f)
+    b = oracle.mktend(*args, *g)
+    ab = oracle.mktend(*args, *[x + 2 * y for x, y in zip(f, g)])
+    for x, y, z in zip(a, b, ab):
+        assert np.abs(z - (x + 2 * y)).max() < 1e-13 * np.abs(z).max()
+    # the divergence part of mktend with only fu=fv inputs equals uv2dv's
+    z0 = [np.zeros_like(f[0])] * 6
+    d, t, z = oracle.mktend(*args, z0[0], f[1], f[2], z0[3], z0[4], z0[5])
+    # mktend(fu, fv): D += -qe fv + i qm fu ; uv2dv(u, v): D = -qe v + i qm u with (u, v) = (fu, fv)
+    pd2, pz2 = oracle.uv2dv(ntru, nlat, nlon, nl, tab["qe"], tab["qm"], one, f[1], f[2])
+    assert np.abs(d - pd2).max() < 1e-13 * np.abs(pd2).max()
+    assert np.abs(z - pz2).max() < 1e-13 * np.abs(pz2).max()
+
+
+def test_minvers_makebm(oracle):
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((10, 10)) + 10 * np.eye(10)
+    assert np.abs(oracle.minvers(a) - np.linalg.inv(a)).max() < 1e-13
+    nlev, ntru = 5, 21
+    t0 = 250 + 10 * rng.random(nlev)
+    ds = np.full(nlev, 1.0 / nlev)
+    g = np.tril(rng.random((nlev, nlev)))
+    tau = rng.random((nlev, nlev))
+    bm1 = oracle.makebm(nlev, ntru, 0.1, t0, ds, g, tau)
+    for jn in (1, 7, 21):
+        # bm1(jlev2, jlev1) = zaq(jlev1, jlev2) = delt^2 (t0(j1) ds(j2) + g(:,j1).tau(j2,:))
+        zaq = np.array([[0.01 * (t0[j1] * ds[j2] + g[:, j1] @ tau[j2, :]) for j1 in range(nlev)]
+                        for j2 in range(nlev)])
+        zaq = zaq + np.eye(nlev) / (jn * (jn + 1))
+        assert np.abs(bm1[jn - 1] - np.linalg.inv(zaq)).max() < 1e-10
