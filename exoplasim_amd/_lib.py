@@ -27,6 +27,12 @@ PROTOTYPES = {
     "exo_fc2sp_f32": (c_int, [_fp, _fp, c_int]),
     "exo_dv2uv_f64": (c_int, [_dp, _dp, _dp, _dp]),
     "exo_dv2uv_f32": (c_int, [_fp, _fp, _fp, _fp]),
+    "exo_uv2dv_f64": (c_int, [_dp, _dp, _dp, _dp]),
+    "exo_uv2dv_f32": (c_int, [_fp, _fp, _fp, _fp]),
+    "exo_qtend_f64": (c_int, [_dp, _dp, _dp, _dp]),
+    "exo_qtend_f32": (c_int, [_fp, _fp, _fp, _fp]),
+    "exo_mktend_f64": (c_int, [_dp] * 9),
+    "exo_mktend_f32": (c_int, [_fp] * 9),
     "exo_gp2fc_f64": (c_int, [_dp, c_int, c_int]),
     "exo_gp2fc_f32": (c_int, [_fp, c_int, c_int]),
     "exo_fc2gp_f64": (c_int, [_dp, c_int, c_int]),

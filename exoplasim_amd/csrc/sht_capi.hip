@@ -468,3 +468,141 @@ extern "C" int exo_gp2fc_f64(double* a, int n, int lot) { return fft_impl<double
 extern "C" int exo_gp2fc_f32(float* a, int n, int lot) { return fft_impl<float>(a, n, lot, -1); }
 extern "C" int exo_fc2gp_f64(double* a, int n, int lot) { return fft_impl<double>(a, n, lot, +1); }
 extern "C" int exo_fc2gp_f32(float* a, int n, int lot) { return fft_impl<float>(a, n, lot, +1); }
+
+// ---------------------------------------------------------------------------
+// multi-input direct transforms (uv2dv, qtend, mktend)
+// ---------------------------------------------------------------------------
+// run one output field of a multi-input direct transform over nlev levels;
+// din[i] are device copies of the inputs
+template <typename R>
+static int dirx_one(exo_sht_plan_s* p, const R* const* din, int nin, R* dout,
+                    std::initializer_list<DirxTerm> terms) {
+  const int mode = sizeof(R) == 8 ? MODE_F64 : MODE_F32;
+  DirxArgs a{};
+  for (int i = 0; i < nin; ++i) a.in[i] = din[i];
+  a.out = dout;
+  a.nin = nin;
+  a.nlev = p->nlev;
+  a.nterm = 0;
+  for (const DirxTerm& t : terms) a.terms[a.nterm++] = t;
+  const size_t need = dirx_scratch_bytes(p->tb, mode, p->nlev);
+  if (need > 0) {
+    int rc = grow(&p->scratch, &p->scratch_bytes, need);
+    if (rc) return rc;
+  }
+  return launch_dirx(p->tb, mode, a, p->scratch, p->scratch_bytes, 0);
+}
+
+// stage nin Fourier inputs and nout spectral outputs (all nlev levels)
+template <typename R, typename Fn>
+static int dirx_staged(const R* const* in, int nin, R* const* out, int nout, Fn run) {
+  if (!g_plan) return EXO_ERR_NOINIT;
+  const Tables& t = g_plan->tb;
+  const long nhor = (long)t.nlat * t.nlon * g_plan->nlev;
+  const long nsp = 2L * t.ncsp * g_plan->nlev;
+  std::vector<R> hin(nhor * nin), hout(nsp * nout);
+  for (int i = 0; i < nin; ++i) {
+    if (!in[i]) return EXO_ERR_ARG;
+    memcpy(hin.data() + i * nhor, in[i], sizeof(R) * nhor);
+  }
+  for (int o = 0; o < nout; ++o)
+    if (!out[o]) return EXO_ERR_ARG;
+  int rc = staged<R>(hin.size(), hin.data(), hout.size(), hout.data(),
+                     [&](exo_sht_plan_s* p, R* din, R* dout) {
+                       const R* ins[3] = {din, din + nhor, din + 2 * nhor};
+                       R* outs[3] = {dout, dout + nsp, dout + 2 * nsp};
+                       return run(p, ins, outs);
+                     });
+  if (rc) return rc;
+  for (int o = 0; o < nout; ++o) memcpy(out[o], hout.data() + o * nsp, sizeof(R) * nsp);
+  return EXO_OK;
+}
+
+// uv2dv(pu, pv, pd, pz), src/legmod.f90:356 (folded branch)
+template <typename R>
+static int uv2dv_impl(const R* pu, const R* pv, R* pd, R* pz) {
+  const R* in[2] = {pu, pv};
+  R* out[2] = {pd, pz};
+  return dirx_staged<R>(in, 2, out, 2, [&](exo_sht_plan_s* p, const R** din, R** dout) {
+    // pd = -qe.v + i qm.u ; pz = qe.u + i qm.v   (inputs 0 = u, 1 = v)
+    int rc = dirx_one<R>(p, din, 2, dout[0],
+                         {{1, 0, 0, -1, TQE}, {0, 1, 0, -1, TQM},
+                          {1, 1, 1, -1, TQE}, {0, 0, 1, +1, TQM}});
+    if (rc) return rc;
+    return dirx_one<R>(p, din, 2, dout[1],
+                       {{0, 0, 0, +1, TQE}, {1, 1, 0, -1, TQM},
+                        {0, 1, 1, +1, TQE}, {1, 0, 1, +1, TQM}});
+  });
+}
+extern "C" int exo_uv2dv_f64(const double* pu, const double* pv, double* pd, double* pz) {
+  return uv2dv_impl<double>(pu, pv, pd, pz);
+}
+extern "C" int exo_uv2dv_f32(const float* pu, const float* pv, float* pd, float* pz) {
+  return uv2dv_impl<float>(pu, pv, pd, pz);
+}
+
+// qtend(q, qn, uq, vq), src/legmod.f90:433 (folded branch)
+template <typename R>
+static int qtend_impl(R* q, const R* qn, const R* uq, const R* vq) {
+  const R* in[3] = {qn, uq, vq};
+  R* out[1] = {q};
+  return dirx_staged<R>(in, 3, out, 1, [&](exo_sht_plan_s* p, const R** din, R** dout) {
+    // q = qe.vq + qc.qn - i qm.uq   (inputs 0 = qn, 1 = uq, 2 = vq)
+    return dirx_one<R>(p, din, 3, dout[0],
+                       {{2, 0, 0, +1, TQE}, {0, 0, 0, +1, TQC}, {1, 1, 0, +1, TQM},
+                        {2, 1, 1, +1, TQE}, {0, 1, 1, +1, TQC}, {1, 0, 1, -1, TQM}});
+  });
+}
+extern "C" int exo_qtend_f64(double* q, const double* qn, const double* uq, const double* vq) {
+  return qtend_impl<double>(q, qn, uq, vq);
+}
+extern "C" int exo_qtend_f32(float* q, const float* qn, const float* uq, const float* vq) {
+  return qtend_impl<float>(q, qn, uq, vq);
+}
+
+// mktend(d, t, z, tn, fu, fv, ke, ut, vt), src/legmod.f90:505 (folded)
+template <typename R>
+static int mktend_impl(R* d, R* t, R* z, const R* tn, const R* fu, const R* fv, const R* ke,
+                       const R* ut, const R* vt) {
+  const R* in_d[3] = {ke, fv, fu};
+  const R* in_t[3] = {vt, tn, ut};
+  const R* in_z[2] = {fu, fv};
+  int rc;
+  {
+    R* out[1] = {d};
+    // D = qq.ke - qe.fv + i qm.fu   (0 = ke, 1 = fv, 2 = fu)
+    rc = dirx_staged<R>(in_d, 3, out, 1, [&](exo_sht_plan_s* p, const R** din, R** dout) {
+      return dirx_one<R>(p, din, 3, dout[0],
+                         {{0, 0, 0, +1, TQQ}, {1, 0, 0, -1, TQE}, {2, 1, 0, -1, TQM},
+                          {0, 1, 1, +1, TQQ}, {1, 1, 1, -1, TQE}, {2, 0, 1, +1, TQM}});
+    });
+    if (rc) return rc;
+  }
+  {
+    R* out[1] = {t};
+    // T = qe.vt + qc.tn - i qm.ut   (0 = vt, 1 = tn, 2 = ut)
+    rc = dirx_staged<R>(in_t, 3, out, 1, [&](exo_sht_plan_s* p, const R** din, R** dout) {
+      return dirx_one<R>(p, din, 3, dout[0],
+                         {{0, 0, 0, +1, TQE}, {1, 0, 0, +1, TQC}, {2, 1, 0, +1, TQM},
+                          {0, 1, 1, +1, TQE}, {1, 1, 1, +1, TQC}, {2, 0, 1, -1, TQM}});
+    });
+    if (rc) return rc;
+  }
+  R* out[1] = {z};
+  // Z = qe.fu + i qm.fv   (0 = fu, 1 = fv)
+  return dirx_staged<R>(in_z, 2, out, 1, [&](exo_sht_plan_s* p, const R** din, R** dout) {
+    return dirx_one<R>(p, din, 2, dout[0],
+                       {{0, 0, 0, +1, TQE}, {1, 1, 0, -1, TQM},
+                        {0, 1, 1, +1, TQE}, {1, 0, 1, +1, TQM}});
+  });
+}
+extern "C" int exo_mktend_f64(double* d, double* t, double* z, const double* tn,
+                              const double* fu, const double* fv, const double* ke,
+                              const double* ut, const double* vt) {
+  return mktend_impl<double>(d, t, z, tn, fu, fv, ke, ut, vt);
+}
+extern "C" int exo_mktend_f32(float* d, float* t, float* z, const float* tn, const float* fu,
+                              const float* fv, const float* ke, const float* ut,
+                              const float* vt) {
+  return mktend_impl<float>(d, t, z, tn, fu, fv, ke, ut, vt);
+}

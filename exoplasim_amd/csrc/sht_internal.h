@@ -62,6 +62,22 @@ struct FwdArgs {
   int table;        // TQC
 };
 
+// One term of a multi-input direct transform: out[oc] += sign * t * X_in[ic]
+struct DirxTerm {
+  int8_t in, ic, oc, sign;
+  int8_t table;
+};
+struct DirxArgs {
+  const void* in[3];  // Fourier inputs fc(2, NLON/2, NLAT, nlev)
+  void* out;          // spectral output (2*NCSP, nlev)
+  int nin, nlev, nterm;
+  DirxTerm terms[8];
+};
+
+int launch_dirx(const Tables& T, int mode, const DirxArgs& a, void* scratch, size_t sbytes,
+                hipStream_t s);
+size_t dirx_scratch_bytes(const Tables& T, int mode, int nlev);
+
 int launch_inverse(const Tables& T, int mode, bool fused, const InvArgs& a, hipStream_t s);
 int launch_forward(const Tables& T, int mode, bool fused, const FwdArgs& a, void* scratch,
                    size_t scratch_bytes, hipStream_t s);

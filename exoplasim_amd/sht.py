@@ -151,6 +151,41 @@ def dv2uv(pd: np.ndarray, pz: np.ndarray):
     return pu, pv
 
 
+def _levels(*arrs):
+    ntru = _need()
+    nlat, nlon, ncsp = dims(ntru)
+    nlev = _state["nlev"]
+    dt = np.asarray(arrs[0]).dtype
+    return [np.ascontiguousarray(a, dtype=dt).reshape(nlev, nlat, nlon) for a in arrs], nlev, ncsp, dt
+
+
+def uv2dv(pu: np.ndarray, pv: np.ndarray):
+    """(u cos, v cos) Fourier coefficients -> (divergence, vorticity)
+    spectral coefficients over NLEV levels (src/legmod.f90:356)."""
+    (pu, pv), nlev, ncsp, dt = _levels(pu, pv)
+    pd = np.zeros((nlev, 2 * ncsp), dtype=dt)
+    pz = np.zeros((nlev, 2 * ncsp), dtype=dt)
+    _call("uv2dv", pu, pv, pd, pz)
+    return pd, pz
+
+
+def qtend(qn, uq, vq):
+    """Tracer tendency q (src/legmod.f90:433)."""
+    (qn, uq, vq), nlev, ncsp, dt = _levels(qn, uq, vq)
+    q = np.zeros((nlev, 2 * ncsp), dtype=dt)
+    _call("qtend", q, qn, uq, vq)
+    return q
+
+
+def mktend(tn, fu, fv, ke, ut, vt):
+    """Spectral tendencies (d, t, z) of the adiabatic terms
+    (src/legmod.f90:505)."""
+    ins, nlev, ncsp, dt = _levels(tn, fu, fv, ke, ut, vt)
+    d, t, z = (np.zeros((nlev, 2 * ncsp), dtype=dt) for _ in range(3))
+    _call("mktend", d, t, z, *ins)
+    return d, t, z
+
+
 def gp2fc(a: np.ndarray) -> np.ndarray:
     """Forward real FFT of rows, returns the transformed copy
     (src/fftmod.f90:29)."""

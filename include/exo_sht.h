@@ -92,6 +92,21 @@ int exo_fc2sp_f32(const float *fc, float *sp, int nfl);
  * coefficient (the caller's array is not modified) */
 int exo_dv2uv_f64(const double *pd, const double *pz, double *pu, double *pv);
 int exo_dv2uv_f32(const float *pd, const float *pz, float *pu, float *pv);
+/* uv2dv (src/legmod.f90:356), folded branch: pu, pv (NHOR x NLEV) Fourier
+ * coefficients of u*cos, v*cos -> pd, pz (NRSP x NLEV) */
+int exo_uv2dv_f64(const double *pu, const double *pv, double *pd, double *pz);
+int exo_uv2dv_f32(const float *pu, const float *pv, float *pd, float *pz);
+/* qtend (src/legmod.f90:433), folded branch: q (NRSP x NLEV) from the
+ * Fourier coefficients qn, uq, vq (NHOR x NLEV) */
+int exo_qtend_f64(double *q, const double *qn, const double *uq, const double *vq);
+int exo_qtend_f32(float *q, const float *qn, const float *uq, const float *vq);
+/* mktend (src/legmod.f90:505), folded branch: spectral tendencies d, t, z
+ * (NRSP x NLEV) from the Fourier coefficients tn, fu, fv, ke, ut, vt
+ * (NHOR x NLEV); the model passes fu <- gvz, fv <- guz (src/plasim.f90:2478) */
+int exo_mktend_f64(double *d, double *t, double *z, const double *tn, const double *fu,
+                   const double *fv, const double *ke, const double *ut, const double *vt);
+int exo_mktend_f32(float *d, float *t, float *z, const float *tn, const float *fu,
+                   const float *fv, const float *ke, const float *ut, const float *vt);
 /* gp2fc / fc2gp (src/fftmod.f90:29,64), in place over lot rows of length n */
 int exo_gp2fc_f64(double *a, int n, int lot);
 int exo_gp2fc_f32(float *a, int n, int lot);

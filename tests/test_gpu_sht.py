@@ -178,3 +178,28 @@ def test_error_paths(gpu):
         sht.Plan(30)
     with pytest.raises(sht.ExoError):
         sht.legini(42, 50, 10)
+
+
+@pytest.mark.parametrize("ntru,nlev,dtype,tol", [(21, 10, np.float64, 1e-13),
+                                                 (42, 20, np.float32, 2e-6),
+                                                 (63, 3, np.float64, 1e-13),
+                                                 (127, 2, np.float32, 2e-6)])
+def test_dropin_multi_direct(oracle, gpu, ntru, nlev, dtype, tol):
+    print(f"uv2dv/qtend/mktend T{ntru} {np.dtype(dtype).name}", flush=True)
+    nlat, nlon, ncsp = sht.dims(ntru)
+    sht.legini(ntru, nlat, nlev)
+    tab = oracle.legini(ntru, nlat)
+    one = np.ones(ntru + 1)
+    rng = np.random.default_rng(ntru)
+    f = [rng.standard_normal((nlev, nlat, nlon)) for _ in range(6)]
+    fd = [x.astype(dtype) for x in f]
+    pd, pz = sht.uv2dv(fd[0], fd[1])
+    od, oz = oracle.uv2dv(ntru, nlat, nlon, nlev, tab["qe"], tab["qm"], one, f[0], f[1])
+    assert rel_l2(pd, od) < tol and rel_l2(pz, oz) < tol
+    q = sht.qtend(fd[0], fd[1], fd[2])
+    oq = oracle.qtend(ntru, nlat, nlon, nlev, tab["qc"], tab["qe"], tab["qm"], one, f[0], f[1], f[2])
+    assert rel_l2(q, oq) < tol
+    d, t, z = sht.mktend(*fd)
+    o = oracle.mktend(ntru, nlat, nlon, nlev, tab["qc"], tab["qe"], tab["qm"], tab["qq"], one, *f)
+    for x, y in zip((d, t, z), o):
+        assert rel_l2(x, y) < tol
