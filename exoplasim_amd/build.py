@@ -44,5 +44,34 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FORTRAN_BIN = os.path.join(FORTRAN, "bin")
+SHIM = ["exo_legmod_capi.f90", "exo_transforms.f90"]
+
+
+def build_fortran_driver(driver: str, verbose: bool = False) -> list:
+    """Compile the Fortran drop-in (exo_legmod_capi.f90 + exo_transforms.f90)
+    with a test program into fortran/bin/driver_r8 (-fdefault-real-8, the
+    MOST_PREC=8 model) and driver_r4, linked to libexosht.so."""
+    fc = os.environ.get("EXO_FC") or shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    os.makedirs(FORTRAN_BIN, exist_ok=True)
+    outs = []
+    for tag, flags in (("r8", ["-fdefault-real-8"]), ("r4", [])):
+        mod = os.path.join(FORTRAN_BIN, f"mod_{tag}")
+        os.makedirs(mod, exist_ok=True)
+        exe = os.path.join(FORTRAN_BIN, f"driver_{tag}")
+        cmd = [fc, "-O2", *flags, f"-J{mod}", *[os.path.join(FORTRAN, s) for s in SHIM],
+               driver, "-o", exe, f"-L{HERE}", "-lexosht", f"-Wl,-rpath,{HERE}",
+               "-Wl,-rpath,$ORIGIN/../.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError("amdflang failed building the Fortran drop-in driver")
+        outs.append(exe)
+    return outs
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))

@@ -1,0 +1,76 @@
+"""The Fortran drop-in on the GPU: a Fortran program calls the reference's
+subroutine names (inigau, sp2fc, sp2fcdmu, fc2sp, dv2uv, uv2dv, qtend,
+mktend, fc2gp, gp2fc) through exoplasim_amd/fortran (ISO_C_BINDING) and
+libexosht.so; its results are compared with the oracle on the same inputs.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(HERE, "golden"))
+from make_golden import golden_input, spectral_input  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(ROOT, "exoplasim_amd", "fortran", "bin")
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+@pytest.mark.parametrize("tag,ntru,nlat,nlev,tol", [("r8", 21, 32, 10, 1e-13),
+                                                    ("r4", 42, 64, 20, 2e-6)])
+def test_fortran_dropin(oracle, gpu, tmp_path, tag, ntru, nlat, nlev, tol):
+    exe = os.path.join(BIN, f"driver_{tag}")
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} not built (exoplasim_amd.build.build_fortran_driver)")
+    out = tmp_path / "out.bin"
+    print(f"fortran driver {tag} T{ntru} L{nlev}", flush=True)
+    res = subprocess.run([exe, str(ntru), str(nlat), str(nlev), str(out)], capture_output=True,
+                         text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    nlon = 2 * nlat
+    ncsp = (ntru + 1) * (ntru + 2) // 2
+    nhor = nlat * nlon
+    v = np.fromfile(out, dtype=np.float64)
+    sizes = [("sid", nlat), ("gwd", nlat), ("fc0", nhor), ("fcd", nhor), ("sp0", 2 * ncsp),
+             ("pu", nhor * nlev), ("pv", nhor * nlev), ("pd", 2 * ncsp * nlev),
+             ("pz", 2 * ncsp * nlev), ("q", 2 * ncsp * nlev), ("d", 2 * ncsp * nlev),
+             ("t", 2 * ncsp * nlev), ("z", 2 * ncsp * nlev), ("g", nhor), ("g2", nhor)]
+    assert v.size == sum(s for _, s in sizes)
+    r, o = {}, 0
+    for name, s in sizes:
+        r[name] = v[o:o + s]
+        o += s
+    sid, gwd = oracle.inigau(nlat)
+    assert np.abs(r["sid"] - sid).max() < 1e-15
+    tab = oracle.legini(ntru, nlat, r["sid"], r["gwd"])
+    one = np.ones(ntru + 1)
+    nfl = 3 * nlev + 1
+    sp = spectral_input(ntru, nfl, 0.3)
+    f = [golden_input((nlev, nlat, nlon), 1.1 + 0.5 * k) for k in range(6)]
+    if tag == "r4":
+        sp = sp.astype(np.float32).astype(np.float64)
+        f = [x.astype(np.float32).astype(np.float64) for x in f]
+    fc0 = oracle.sp2fc(ntru, nlat, nlon, tab["qi"], one, sp[0])
+    assert rel_l2(r["fc0"], fc0) < tol
+    assert rel_l2(r["fcd"], oracle.sp2fc(ntru, nlat, nlon, tab["qj"], one, sp[-1])) < tol
+    assert rel_l2(r["sp0"], oracle.fc2sp(ntru, nlat, nlon, tab["qc"], one, r["fc0"])) < tol
+    pu, pv = oracle.dv2uv(ntru, nlat, nlon, nlev, tab["qu"], tab["qv"], one, 1.63299310207,
+                          sp[nlev:2 * nlev], sp[:nlev])
+    assert rel_l2(r["pu"], pu) < tol and rel_l2(r["pv"], pv) < tol
+    pd, pz = oracle.uv2dv(ntru, nlat, nlon, nlev, tab["qe"], tab["qm"], one, f[0], f[1])
+    assert rel_l2(r["pd"], pd) < tol and rel_l2(r["pz"], pz) < tol
+    q = oracle.qtend(ntru, nlat, nlon, nlev, tab["qc"], tab["qe"], tab["qm"], one, f[0], f[1], f[2])
+    assert rel_l2(r["q"], q) < tol
+    d, t, z = oracle.mktend(ntru, nlat, nlon, nlev, tab["qc"], tab["qe"], tab["qm"], tab["qq"],
+                            one, *f)
+    assert rel_l2(r["d"], d) < tol and rel_l2(r["t"], t) < tol and rel_l2(r["z"], z) < tol
+    g = oracle.fc2gp(r["fc0"].reshape(nlat, nlon))
+    assert rel_l2(r["g"], g) < tol
+    assert rel_l2(r["g2"], oracle.gp2fc(r["g"].reshape(nlat, nlon))) < tol
