@@ -89,6 +89,22 @@ void ora_makebm(int nlev, int ntru, double delt, const double *t0,
                 const double *dsigma, const double *g, const double *tau,
                 double *bm1);
 
+/* ---- long-wave radiation column (src/radmod.f90:2122-2456) ----
+ * arrays [col][lev]; dt has nlev+1 values per column (surface last),
+ * dq/dqo3/dqco2/dcc/dql nlev; dftu/dftd out nlev+1 */
+void ora_lwr(int nlev, int ncol, const double *sigma, const double *sigmah,
+             const double *dsigma, double ga, double th2oc, double tpofmt,
+             double acllwr, double clgray, int npbroaden, const double *dp,
+             const double *dt, const double *dq, const double *dqo3,
+             const double *dqco2, const double *dcc, const double *dql,
+             const double *dls, double *dftu, double *dftd);
+/* heating rates in K/s (src/radmod.f90:1107-1108) */
+void ora_lwr_heating(int nlev, int ncol, const double *dsigma, double ga, double acpd,
+                     double adv, const double *dp, const double *dq, const double *dftu,
+                     const double *dftd, double *dtdt);
+/* sigma levels for neqsig = 0 (src/plasim.f90:1553-1567) */
+void ora_sigma_levels(int nlev, double *sigma, double *sigmah, double *dsigma);
+
 #ifdef __cplusplus
 }
 #endif

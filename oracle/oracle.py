@@ -344,3 +344,79 @@ class RefFft991:
         n = a.shape[-1]
         self.lib.fc2gp_(_p(a), byref(c_int(n)), byref(c_int(a.size // n)))
         return a
+
+
+# ---------------------------------------------------------------------------
+# long-wave radiation column (src/radmod.f90:2122-2456, 1100-1108)
+# ---------------------------------------------------------------------------
+# physical constants of the Earth setup (src/p_earth.f90:41-58,
+# src/plasim.f90:1317-1318, src/plasimmod.f90:122)
+GA = 9.80665
+GASCON = 287.0
+AKAP = 0.286
+ACPD = GASCON / AKAP
+ADV = 1870.0 / ACPD - 1.0
+# radmod namelist defaults (src/radmod.f90:37-43,71)
+LWR_DEFAULTS = dict(th2oc=0.024, tpofmt=1.0, acllwr=0.1, clgray=-1.0, npbroaden=1)
+
+
+def sigma_levels(oracle_lib, nlev):
+    s = np.zeros(nlev)
+    sh = np.zeros(nlev)
+    ds = np.zeros(nlev)
+    oracle_lib.ora_sigma_levels(c_int(nlev), _p(s), _p(sh), _p(ds))
+    return s, sh, ds
+
+
+def lwr(ora: "Oracle", cols: dict, nlev: int, **params):
+    """Run the oracle lwr on columns.  cols: dt (ncol, nlev+1), dq, dqo3, dqco2,
+    dcc, dql (ncol, nlev), dp, dls (ncol).  Returns dftu, dftd (ncol, nlev+1),
+    heating (ncol, nlev) in K/s and OLR (ncol)."""
+    p = dict(LWR_DEFAULTS)
+    p.update(params)
+    s, sh, ds = sigma_levels(ora.lib, nlev)
+    ncol = cols["dp"].shape[0]
+    c = {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in cols.items()}
+    dftu = np.zeros((ncol, nlev + 1))
+    dftd = np.zeros((ncol, nlev + 1))
+    ora.lib.ora_lwr(c_int(nlev), c_int(ncol), _p(s), _p(sh), _p(ds), c_double(GA),
+                    c_double(p["th2oc"]), c_double(p["tpofmt"]), c_double(p["acllwr"]),
+                    c_double(p["clgray"]), c_int(int(p["npbroaden"])), _p(c["dp"]), _p(c["dt"]),
+                    _p(c["dq"]), _p(c["dqo3"]), _p(c["dqco2"]), _p(c["dcc"]), _p(c["dql"]),
+                    _p(c["dls"]), _p(dftu), _p(dftd))
+    heat = np.zeros((ncol, nlev))
+    ora.lib.ora_lwr_heating(c_int(nlev), c_int(ncol), _p(ds), c_double(GA), c_double(ACPD),
+                            c_double(ADV), _p(c["dp"]), _p(c["dq"]), _p(dftu), _p(dftd),
+                            _p(heat))
+    return dftu, dftd, heat, -dftu[:, 0]
+
+
+class RefRad:
+    """The reference's lwr at its default resolution module (NHOR=2048
+    columns, NLEV=10), FP64; loaded with lazy binding because radmod's other
+    routines reference model subroutines that this library does not link."""
+
+    def __init__(self):
+        self.lib = ctypes.CDLL(_ref_path("libref_rad_t21.so"), mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+        d = (c_int * 2)()
+        self.lib.ref_rad_dims(d)
+        self.nhor, self.nlev = list(d)
+
+    def lwr(self, cols: dict, **params):
+        p = dict(LWR_DEFAULTS)
+        p.update(params)
+        nl, nh = self.nlev, self.nhor
+        s, sh, ds = sigma_levels(Oracle().lib, nl)
+        f = {k: np.asfortranarray(np.asarray(v, dtype=np.float64)) for k, v in cols.items()}
+        # (ncol, lev) arrays are passed as Fortran (NHOR, lev): transpose to C order
+        t = {k: np.ascontiguousarray(v.T) if v.ndim == 2 else np.ascontiguousarray(v)
+             for k, v in f.items()}
+        ftu = np.zeros((nl + 1, nh))
+        ftd = np.zeros((nl + 1, nh))
+        dbl = lambda x: byref(c_double(x))  # noqa: E731
+        self.lib.ref_rad_lwr(_p(s), _p(sh), _p(ds), dbl(GA), dbl(p["th2oc"]), dbl(p["tpofmt"]),
+                             dbl(p["acllwr"]), dbl(p["clgray"]), c_int(int(p["npbroaden"])),
+                             _p(t["dp"]), _p(t["dt"]), _p(t["dq"]), _p(t["dqo3"]),
+                             _p(t["dqco2"]), _p(t["dcc"]), _p(t["dql"]), _p(t["dls"]),
+                             _p(ftu), _p(ftd))
+        return ftu.T.copy(), ftd.T.copy()
