@@ -20,7 +20,9 @@ BIN = os.path.join(ROOT, "exoplasim_amd", "fortran", "bin")
 
 
 def rel_l2(a, b):
-    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+    a = np.ravel(a)
+    b = np.ravel(b)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
 @pytest.mark.parametrize("tag,ntru,nlat,nlev,tol", [("r8", 21, 32, 10, 1e-13),
