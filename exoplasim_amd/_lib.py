@@ -50,6 +50,11 @@ PROTOTYPES = {
                                c_long, c_int, c_int, c_void_p]),
     "exo_sht_roundtrip": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p,
                                   c_void_p, c_void_p]),
+    "exo_si_create": (c_int, [POINTER(c_void_p), c_int, c_int, _dp, _dp, _dp, _dp]),
+    "exo_si_destroy": (c_int, [c_void_p]),
+    "exo_si_makebm": (c_int, [c_void_p, c_double, c_void_p]),
+    "exo_si_bm1": (c_int, [c_void_p, _dp]),
+    "exo_si_apply": (c_int, [c_void_p, c_int, c_double] + [c_void_p] * 8),
 }
 
 _lib = None

@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libexosht.so")
 
-SOURCES = ["sht_tables.hip", "sht_transforms.hip", "sht_direct_terms.hip", "sht_capi.hip"]
+SOURCES = ["sht_tables.hip", "sht_transforms.hip", "sht_direct_terms.hip", "sht_capi.hip",
+           "si_solve.hip"]
 HIPFLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions are kept out of every kernel

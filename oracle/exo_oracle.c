@@ -1101,3 +1101,62 @@ void ora_makebm(int nlev, int ntru, double delt, const double *t0,
     ora_minvers(b, nlev);
   }
 }
+
+/* semi-implicit step of spectrala, sections 1-3 (src/plasim.f90:2817-2876),
+ * one process (NSPP = nsp): overwrites sdt with the divergence at time t,
+ * updates spt and stt.  g, tau, bm1 column-major; spectral arrays
+ * (nsp, nlev) column-major; nindex(jsp) = total wavenumber. */
+void ora_si_apply(int nlev, int nsp, const int *nindex, double delt, const double *t0,
+                  const double *dsigma, const double *g, const double *tau,
+                  const double *bm1, const double *sop, const double *spm,
+                  const double *atm, const double *adm, double *stt, double *sdt,
+                  double *spt) {
+  double *zgt = (double *)malloc(sizeof(double) * nsp * nlev);
+  double *zgm = (double *)malloc(sizeof(double) * nsp * nlev);
+  double *adt = (double *)malloc(sizeof(double) * nsp * nlev);
+  memcpy(adt, sdt, sizeof(double) * nsp * nlev);
+  /* 1.a */
+  for (int jlev = 0; jlev < nlev; ++jlev)
+    for (int jsp = 0; jsp < nsp; ++jsp) {
+      double a = 0.0, b = 0.0;
+      for (int k = 0; k < nlev; ++k) {
+        a += g[jlev * nlev + k] * stt[k * nsp + jsp];
+        b += g[jlev * nlev + k] * atm[k * nsp + jsp];
+      }
+      zgt[jlev * nsp + jsp] = a;
+      zgm[jlev * nsp + jsp] = b;
+    }
+  /* 1.b */
+  for (int jlev = 0; jlev < nlev; ++jlev)
+    for (int jsp = 0; jsp < nsp; ++jsp) {
+      const int jn = nindex[jsp];
+      double zsum = 0.0;
+      if (jn > 0) {
+        const double zq = 1.0 / (double)(jn * (jn + 1));
+        for (int j2 = 0; j2 < nlev; ++j2) {
+          const double z0 = t0[j2];
+          const double zt = zgt[j2 * nsp + jsp] - z0 * spt[jsp];
+          const double zm = zgm[j2 * nsp + jsp] + z0 * spm[jsp];
+          const double za = adt[j2 * nsp + jsp] * zq + sop[jsp];
+          const double zb = adm[j2 * nsp + jsp] * zq;
+          const double zz = zb + delt * (zm + za + delt * zt);
+          zsum = zsum + zz * bm1[(long)(jn - 1) * nlev * nlev + jlev * nlev + j2];
+        }
+      }
+      sdt[jlev * nsp + jsp] = zsum;
+    }
+  for (int jlev = 0; jlev < nlev; ++jlev) sdt[jlev * nsp] = sdt[jlev * nsp + 1] = 0.0;
+  /* 2 */
+  for (int jlev = 0; jlev < nlev; ++jlev)
+    for (int jsp = 0; jsp < nsp; ++jsp) spt[jsp] = spt[jsp] + dsigma[jlev] * sdt[jlev * nsp + jsp];
+  /* 3 */
+  for (int jlev = 0; jlev < nlev; ++jlev)
+    for (int jsp = 0; jsp < nsp; ++jsp) {
+      double s = 0.0;
+      for (int k = 0; k < nlev; ++k) s += tau[jlev * nlev + k] * sdt[k * nsp + jsp];
+      stt[jlev * nsp + jsp] = stt[jlev * nsp + jsp] - s;
+    }
+  free(zgt);
+  free(zgm);
+  free(adt);
+}

@@ -88,6 +88,11 @@ int ora_minvers(double *a, int n);
 void ora_makebm(int nlev, int ntru, double delt, const double *t0,
                 const double *dsigma, const double *g, const double *tau,
                 double *bm1);
+void ora_si_apply(int nlev, int nsp, const int *nindex, double delt, const double *t0,
+                  const double *dsigma, const double *g, const double *tau,
+                  const double *bm1, const double *sop, const double *spm,
+                  const double *atm, const double *adm, double *stt, double *sdt,
+                  double *spt);
 
 /* ---- long-wave radiation column (src/radmod.f90:2122-2456) ----
  * arrays [col][lev]; dt has nlev+1 values per column (surface last),

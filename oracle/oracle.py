@@ -190,6 +190,24 @@ class Oracle:
         self.lib.ora_minvers(_p(flat), c_int(a.shape[0]))
         return flat.T.copy()
 
+    def si_apply(self, ntru, nlev, delt, t0, dsigma, g, tau, bm1, sop, spm, atm, adm, stt,
+                 sdt, spt):
+        """spectrala's semi-implicit step; g, tau (nlev, nlev) math order;
+        bm1 (ntru, nlev, nlev) as returned by makebm; spectral arrays
+        (nlev, nsp) (level-major).  Returns (stt, sdt, spt) updated copies."""
+        nsp = (ntru + 1) * (ntru + 2)
+        ni = np.concatenate([[n, n] for m in range(ntru + 1) for n in range(m, ntru + 1)])
+        ni = np.ascontiguousarray(ni, dtype=np.int32)
+        c = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+        gcm, tcm = c(np.asarray(g).T), c(np.asarray(tau).T)
+        bcm = c(np.transpose(bm1, (0, 2, 1)))
+        stt, sdt, spt = c(stt).copy(), c(sdt).copy(), c(spt).copy()
+        self.lib.ora_si_apply(c_int(nlev), c_int(nsp), ni.ctypes.data_as(POINTER(c_int)),
+                              c_double(delt), _p(c(t0)), _p(c(dsigma)), _p(gcm), _p(tcm),
+                              _p(bcm), _p(c(sop)), _p(c(spm)), _p(c(atm)), _p(c(adm)),
+                              _p(stt), _p(sdt), _p(spt))
+        return stt, sdt, spt
+
     def makebm(self, nlev, ntru, delt, t0, dsigma, g, tau):
         """g, tau given as (nlev, nlev) numpy arrays in math (row, col) order."""
         gcm = np.ascontiguousarray(np.asarray(g, dtype=np.float64).T)

@@ -6,7 +6,7 @@ import re
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "exo_sht.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("exo_sht.h", "exo_si.h")]
 
 
 def declared_symbols(path):
@@ -19,7 +19,7 @@ def test_header_symbols_exported():
     from exoplasim_amd import _lib
 
     lib = _lib.lib()
-    syms = declared_symbols(HEADER)
+    syms = sorted(set(sum((declared_symbols(h) for h in HEADERS), [])))
     assert len(syms) >= 20
     missing = [s for s in syms if not hasattr(lib, s)]
     assert not missing, missing

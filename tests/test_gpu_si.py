@@ -1,0 +1,40 @@
+"""The semi-implicit solve on the GPU (FP64) against the oracle."""
+import sys
+import os
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+from test_oracle import _si_case  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("ntru,nlev,dtype,tol", [(21, 10, "float64", 1e-12),
+                                                 (42, 20, "float64", 1e-12),
+                                                 (127, 40, "float32", 1e-6)])
+def test_semi_implicit(oracle, gpu, ntru, nlev, dtype, tol):
+    import torch
+
+    from exoplasim_amd.semi_implicit import SemiImplicit
+
+    print(f"semi-implicit T{ntru} L{nlev} {dtype}", flush=True)
+    c = _si_case(ntru, nlev, seed=ntru)
+    si = SemiImplicit(nlev, ntru, c["t0"], c["dsigma"], c["g"], c["tau"])
+    si.makebm(c["delt"])
+    bm1 = si.bm1()
+    ref = oracle.makebm(nlev, ntru, c["delt"], c["t0"], c["dsigma"], c["g"], c["tau"])
+    assert np.abs(bm1 - ref).max() / np.abs(ref).max() < 1e-12
+    dt = getattr(torch, dtype)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dt).cuda()  # noqa: E731
+    t = {k: dev(c[k]) for k in ("sop", "spm", "atm", "adm", "stt", "sdt", "spt")}
+    si.apply(c["delt"], t["sop"], t["spm"], t["atm"], t["adm"], t["stt"], t["sdt"], t["spt"])
+    torch.cuda.synchronize()
+    cast = (lambda a: a.astype(np.float32).astype(np.float64)) if dtype == "float32" else (lambda a: a)
+    o = oracle.si_apply(ntru, nlev, c["delt"], c["t0"], c["dsigma"], c["g"], c["tau"], ref,
+                        *[cast(c[k]) for k in ("sop", "spm", "atm", "adm", "stt", "sdt", "spt")])
+    for got, want in zip((t["stt"], t["sdt"], t["spt"]), o):
+        g = got.double().cpu().numpy()
+        assert np.linalg.norm(g - want) / np.linalg.norm(want) < tol
+    si.close()

@@ -285,3 +285,43 @@ def test_minvers_makebm(oracle):
                         for j2 in range(nlev)])
         zaq = zaq + np.eye(nlev) / (jn * (jn + 1))
         assert np.abs(bm1[jn - 1] - np.linalg.inv(zaq)).max() < 1e-10
+
+
+def _si_case(ntru=21, nlev=6, seed=9):
+    rng = np.random.default_rng(seed)
+    t0 = 250 + 20 * rng.random(nlev)
+    ds = np.full(nlev, 1.0 / nlev)
+    g = np.tril(rng.random((nlev, nlev))) + np.eye(nlev)
+    tau = 0.1 * rng.random((nlev, nlev))
+    nsp = (ntru + 1) * (ntru + 2)
+    arr = lambda *s: rng.standard_normal(s)  # noqa: E731
+    return dict(ntru=ntru, nlev=nlev, delt=0.05, t0=t0, dsigma=ds, g=g, tau=tau,
+                sop=arr(nsp), spm=arr(nsp), atm=arr(nlev, nsp), adm=arr(nlev, nsp),
+                stt=arr(nlev, nsp), sdt=arr(nlev, nsp), spt=arr(nsp))
+
+
+def test_si_apply_solves_eq17(oracle):
+    """the divergence returned solves A^T d = rhs for each spectral real,
+    where A = delt^2 (t0 dsigma + g.tau) + I/(n(n+1)) is the matrix bm1
+    inverts (src/plasim.f90:1683-1703, 2841-2860)."""
+    c = _si_case()
+    ntru, nlev, delt = c["ntru"], c["nlev"], c["delt"]
+    bm1 = oracle.makebm(nlev, ntru, delt, c["t0"], c["dsigma"], c["g"], c["tau"])
+    stt, sdt, spt = oracle.si_apply(ntru, nlev, delt, c["t0"], c["dsigma"], c["g"], c["tau"],
+                                    bm1, c["sop"], c["spm"], c["atm"], c["adm"], c["stt"],
+                                    c["sdt"], c["spt"])
+    ni = np.concatenate([[n, n] for m in range(ntru + 1) for n in range(m, ntru + 1)])
+    g, tau, t0, ds = c["g"], c["tau"], c["t0"], c["dsigma"]
+    zaq = delt ** 2 * (np.outer(ds, t0) + np.array(
+        [[g[:, j1] @ tau[j2, :] for j1 in range(nlev)] for j2 in range(nlev)]))
+    for jsp in range(2, ni.size, 7):
+        n = ni[jsp]
+        a = zaq + np.eye(nlev) / (n * (n + 1))
+        zq = 1.0 / (n * (n + 1))
+        rhs = (c["adm"][:, jsp] * zq + delt * ((g.T @ c["atm"][:, jsp] + t0 * c["spm"][jsp])
+               + c["sdt"][:, jsp] * zq + c["sop"][jsp]
+               + delt * (g.T @ c["stt"][:, jsp] - t0 * c["spt"][jsp])))
+        assert np.allclose(a.T @ sdt[:, jsp], rhs, rtol=1e-10, atol=1e-12)
+    assert np.all(sdt[:, :2] == 0.0)
+    assert np.allclose(spt, c["spt"] + ds @ sdt, rtol=1e-13, atol=1e-13)
+    assert np.allclose(stt, c["stt"] - tau.T @ sdt, rtol=1e-12, atol=1e-12)
