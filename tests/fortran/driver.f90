@@ -1,7 +1,7 @@
 ! driver.f90 -- test program of the Fortran drop-in (exoplasim_amd/fortran):
 ! calls the reference-named subroutines (inigau, sp2fc, sp2fcdmu, fc2sp,
 ! dv2uv, uv2dv, qtend, mktend, fc2gp, gp2fc) on deterministic inputs and
-! writes every result as real(8) to a stream file that
+! writes its inputs and every result as real(8) to a stream file that
 ! tests/test_gpu_fortran.py compares with the oracle.
 !   usage: driver NTRU NLAT NLEV outfile
 program exo_driver
@@ -64,6 +64,7 @@ program exo_driver
   call fc2gp(g, nlon, nlat)
 
   open(10, file=trim(outfile), access='stream', form='unformatted', status='replace')
+  write(10) real(sp, 8), real(f, 8)
   write(10) sid, gwd
   write(10) real(fc0, 8), real(fcd, 8), real(sp0, 8), real(pu, 8), real(pv, 8)
   write(10) real(pd, 8), real(pz, 8), real(q, 8), real(d, 8), real(t, 8), real(z, 8)

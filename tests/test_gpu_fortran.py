@@ -40,7 +40,8 @@ def test_fortran_dropin(oracle, gpu, tmp_path, tag, ntru, nlat, nlev, tol):
     ncsp = (ntru + 1) * (ntru + 2) // 2
     nhor = nlat * nlon
     v = np.fromfile(out, dtype=np.float64)
-    sizes = [("sid", nlat), ("gwd", nlat), ("fc0", nhor), ("fcd", nhor), ("sp0", 2 * ncsp),
+    nfl = 3 * nlev + 1
+    sizes = [("sp", 2 * ncsp * nfl), ("f", nhor * nlev * 6), ("sid", nlat), ("gwd", nlat), ("fc0", nhor), ("fcd", nhor), ("sp0", 2 * ncsp),
              ("pu", nhor * nlev), ("pv", nhor * nlev), ("pd", 2 * ncsp * nlev),
              ("pz", 2 * ncsp * nlev), ("q", 2 * ncsp * nlev), ("d", 2 * ncsp * nlev),
              ("t", 2 * ncsp * nlev), ("z", 2 * ncsp * nlev), ("g", nhor), ("g2", nhor)]
@@ -53,12 +54,11 @@ def test_fortran_dropin(oracle, gpu, tmp_path, tag, ntru, nlat, nlev, tol):
     assert np.abs(r["sid"] - sid).max() < 1e-15
     tab = oracle.legini(ntru, nlat, r["sid"], r["gwd"])
     one = np.ones(ntru + 1)
-    nfl = 3 * nlev + 1
-    sp = spectral_input(ntru, nfl, 0.3)
-    f = [golden_input((nlev, nlat, nlon), 1.1 + 0.5 * k) for k in range(6)]
-    if tag == "r4":
-        sp = sp.astype(np.float32).astype(np.float64)
-        f = [x.astype(np.float32).astype(np.float64) for x in f]
+    # the inputs the program used (its sin/cos/** round differently from numpy)
+    sp = r["sp"].reshape(nfl, 2 * ncsp)
+    f = list(r["f"].reshape(6, nlev, nlat, nlon))
+    assert np.abs(sp - spectral_input(ntru, nfl, 0.3)).max() < 1e-6
+    assert np.abs(f[1] - golden_input((nlev, nlat, nlon), 1.6)).max() < 1e-6
     fc0 = oracle.sp2fc(ntru, nlat, nlon, tab["qi"], one, sp[0])
     assert rel_l2(r["fc0"], fc0) < tol
     assert rel_l2(r["fcd"], oracle.sp2fc(ntru, nlat, nlon, tab["qj"], one, sp[-1])) < tol
