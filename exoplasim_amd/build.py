@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libexosht.so")
 
 SOURCES = ["sht_tables.hip", "sht_transforms.hip", "sht_direct_terms.hip", "sht_capi.hip",
-           "si_solve.hip"]
+           "si_solve.hip", "rad_lwr.hip", "rad_mlp.hip"]
 HIPFLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions are kept out of every kernel

@@ -1,0 +1,229 @@
+// rad_lwr.hip -- the long-wave radiation column of the reference
+// (src/radmod.f90:2122-2456, subroutine lwr) and its heating rate
+// (src/radmod.f90:1100-1108) as a GPU kernel: one thread per column.
+//
+// Arrays use the reference layout field(NHOR, lev): the column index is
+// fastest, so consecutive threads read consecutive addresses.  The column
+// work is O(NLEV^2) pow/log evaluations (absorber paths from every level to
+// every level below), so this kernel is bound by the transcendental
+// throughput of the vector ALUs.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "../../include/exo_rad.h"
+
+namespace {
+
+constexpr int MAXLEV = 64;
+
+template <typename R>
+struct LwrArgs {
+  int nlev, ncol;
+  const double *sigma, *sigmah, *dsigma;
+  double ga, th2oc, tpofmt, acllwr, clgray, acpd, adv;
+  int npbroaden;
+  const R *dp, *dt, *dq, *dqo3, *dqco2, *dcc, *dql, *dls;
+  R *dftu, *dftd, *heat, *olr;  // heat in K/day; any output may be null
+};
+
+template <typename R>
+__device__ __forceinline__ R pw(R x, R y) {
+  if constexpr (std::is_same<R, double>::value) return pow(x, y); else return powf(x, y);
+}
+template <typename R>
+__device__ __forceinline__ R lg10(R x) {
+  if constexpr (std::is_same<R, double>::value) return log10(x); else return log10f(x);
+}
+template <typename R>
+__device__ __forceinline__ R ln(R x) {
+  if constexpr (std::is_same<R, double>::value) return log(x); else return logf(x);
+}
+template <typename R>
+__device__ __forceinline__ R ex(R x) {
+  if constexpr (std::is_same<R, double>::value) return exp(x); else return expf(x);
+}
+
+template <typename R>
+__global__ __launch_bounds__(128) void k_lwr(LwrArgs<R> a) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.ncol) return;
+  const int NLEV = a.nlev, NLEP = a.nlev + 1, NLEM = a.nlev - 1, NC = a.ncol;
+  // 1-based level arrays (index 0 used by zbd only)
+  R zbu[MAXLEV + 2], zbd[MAXLEV + 2], zst4[MAXLEV + 2], zst4h[MAXLEV + 2];
+  R zq[MAXLEV + 1], zqo3[MAXLEV + 1], zqco2[MAXLEV + 1], ztaucc0[MAXLEV + 1];
+  R ztaucs[MAXLEV + 1], ztau[MAXLEV + 1], ztausf[MAXLEV + 1];
+  R dftu[MAXLEV + 2], dftd[MAXLEV + 2];
+  const double* S = a.sigma - 1;
+  const double* SH = a.sigmah - 1;
+  const double* DS = a.dsigma - 1;
+  const R SBK = R(5.67e-8);
+  const R zero = R(1.e-6);
+  const R zpv2pm = R(0.0440098 / 0.0289644);
+  const R zfh2o = R(0.1), zfco2 = R(100. / 1.9635), zfo3 = R(100. / 2.14);
+  const R zao30 = R(0.209) * pw(R(7.e-5), R(0.436));
+  const R zco20 = R(0.0676) * pw(R(0.01022), R(0.421));
+  const R zh2o0a = R(0.846) * pw(R(3.59e-5), R(0.243));
+  const R zh2o0 = R(0.832) * pw(R(0.0286), R(0.26));
+  const R zao3c = R(0.209) * pw(R(0.01 + 7.e-5), R(0.436)) - zao30 - R(0.0212) * lg10(R(0.01));
+  const R zaco2c = R(0.0676) * pw(R(1.01022), R(0.421)) - zco20;
+  const R zah2oc = R(0.846) * pw(R(0.01 + 3.59e-5), R(0.243)) - zh2o0a - R(0.24) * lg10(R(0.02));
+  const R zth2oc = R(1.) - (R(0.832) * pw(R(2. + 0.0286), R(0.26)) - zh2o0) +
+                   R(0.1196) * ln(R(2. - 0.6931));
+  const R ps = a.dp[c];
+  const R zps2 = (a.npbroaden > 0.5) ? ps * ps : R(101100.0) * R(101100.0);
+  for (int j = 1; j <= NLEP; ++j) {
+    dftu[j] = 0;
+    dftd[j] = 0;
+    const R t = a.dt[(long)(j - 1) * NC + c];
+    zst4[j] = SBK * (t * t) * (t * t);
+  }
+  zst4h[1] = zst4[1] - (zst4[1] - zst4[2]) * R(S[1]) / R(S[1] - S[2]);
+  for (int j = 2; j <= NLEV; ++j) {
+    const int jm = j - 1;
+    zst4h[j] = (zst4[j] * R(S[jm] - SH[jm]) + zst4[jm] * R(SH[jm] - S[j])) / R(S[jm] - S[j]);
+  }
+  if ((zst4[NLEV] - zst4h[NLEV]) * (zst4[NLEV] - zst4[NLEP]) > 0)
+    zst4h[NLEP] = zst4[NLEP];
+  else
+    zst4h[NLEP] = zst4[NLEV] + (zst4[NLEV] - zst4[NLEM]) * R(1. - S[NLEV]) / R(S[NLEV] - S[NLEM]);
+  zbd[0] = 0;  // SBK * zttop**4 with zttop = 0
+  const R ls = a.dls[c];
+  R zeps = ls + R(0.98) * (R(1.) - ls);
+  zbu[NLEP] = zeps * zst4[NLEP];
+  for (int j = 1; j <= NLEV; ++j) {
+    const long o = (long)(j - 1) * NC + c;
+    const R zzf1 = R(S[j] * DS[j] / a.ga / 100000.);
+    const R zzf3 = R(-1.66 * a.acllwr * 1000. * DS[j] / a.ga);
+    const R zsfac = zzf1 * zps2;
+    zq[j] = zfh2o * zsfac * a.dq[o];
+    zqo3[j] = zfo3 * zsfac * a.dqo3[o];
+    zqco2[j] = zfco2 * (zpv2pm * R(1.e-6)) * zsfac * a.dqco2[o];
+    if (a.clgray > 0)
+      ztaucc0[j] = R(1.) - a.dcc[o] * R(a.clgray);
+    else
+      ztaucc0[j] = R(1.) - a.dcc[o] * (R(1.) - ex(zzf3 * a.dql[o] * ps));
+  }
+  for (int jlev = 1; jlev <= NLEV; ++jlev) {
+    const int jlem = jlev - 1;
+    R ztaucc = 1, zsumwv = 0, zsumo3 = 0, zsumco2 = 0;
+    for (int j2 = jlev; j2 <= NLEV; ++j2) {
+      zsumwv += zq[j2];
+      zsumo3 += zqo3[j2];
+      zsumco2 += zqco2[j2];
+      R zah2o, zaco2, zth2o, zao3;
+      if (zsumwv <= R(0.01))
+        zah2o = R(0.846) * pw(zsumwv + R(3.59e-5), R(0.243)) - zh2o0a;
+      else
+        zah2o = R(0.24) * lg10(zsumwv + R(0.01)) + zah2oc;
+      if (a.th2oc > 0.) zah2o = fmin(zah2o + (R(1.) - ex(-R(a.th2oc) * zsumwv)), R(1.));
+      if (zsumco2 <= R(1.0))
+        zaco2 = R(0.0676) * pw(zsumco2 + R(0.01022), R(0.421)) - zco20;
+      else
+        zaco2 = R(0.0546) * lg10(zsumco2) + zaco2c;
+      if (zsumwv <= R(2.))
+        zth2o = R(1.) - (R(0.832) * pw(zsumwv + R(0.0286), R(0.26)) - zh2o0);
+      else
+        zth2o = fmax(R(0.), zth2oc - R(0.1196) * ln(zsumwv - R(0.6931)));
+      if (zsumo3 <= R(0.01))
+        zao3 = R(0.209) * pw(zsumo3 + R(7.e-5), R(0.436)) - zao30;
+      else
+        zao3 = R(0.0212) * lg10(zsumo3) + zao3c;
+      R tcs = R(1.) - zah2o - zao3 - zaco2 * zth2o;
+      tcs = fmin(R(1.) - zero, fmax(zero, tcs));
+      ztaucs[j2] = tcs;
+      ztaucc = ztaucc * ztaucc0[j2];
+      ztau[j2] = tcs * ztaucc;
+    }
+    if (jlev == 1) {
+      R ztau0 = R(1.) - zero;
+      for (int j2 = 1; j2 <= NLEV; ++j2) {
+        const int jp2 = j2 + 1;
+        ztau0 = ztaucs[j2] / ztau0 * ztaucc0[j2] * R(a.tpofmt);
+        ztau0 = fmin(R(1.) - zero, fmax(zero, ztau0));
+        if ((zst4[j2] - zst4h[j2]) * (zst4[j2] - zst4h[jp2]) > 0) {
+          zbd[j2] = R(0.5) * zst4[j2] + R(0.25) * (zst4h[j2] + zst4h[jp2]);
+          zbu[j2] = zbd[j2];
+        } else {
+          zbd[j2] = (zst4h[jp2] - ztau0 * zst4h[j2]) / (R(1.) - ztau0) -
+                    (zst4h[j2] - zst4h[jp2]) / ln(ztau0);
+          zbu[j2] = zst4h[j2] + zst4h[jp2] - zbd[j2];
+        }
+        ztau0 = ztaucs[j2];
+      }
+    }
+    dftu[jlev] = dftu[jlev] - zbu[jlev];
+    dftd[jlev] = dftd[jlev] + zbd[jlem];
+    const R zbdl = zbd[jlem] - zbd[jlev];
+    for (int j2 = jlev; j2 <= NLEV; ++j2) {
+      const int jp2 = j2 + 1;
+      dftu[jlev] = dftu[jlev] - (zbu[jp2] - zbu[j2]) * ztau[j2];
+      dftd[jp2] = dftd[jp2] + zbdl * ztau[j2];
+    }
+    ztausf[jlev] = ztau[NLEV];
+  }
+  dftu[NLEP] = dftu[NLEP] - zbu[NLEP];
+  dftd[NLEP] = dftd[NLEP] + zbd[NLEV];
+  zeps = (R(1.) - zeps) * dftd[NLEP];
+  for (int j = 1; j <= NLEV; ++j) dftu[j] = dftu[j] - ztausf[j] * zeps;
+  if (a.dftu)
+    for (int j = 1; j <= NLEP; ++j) a.dftu[(long)(j - 1) * NC + c] = dftu[j];
+  if (a.dftd)
+    for (int j = 1; j <= NLEP; ++j) a.dftd[(long)(j - 1) * NC + c] = dftd[j];
+  if (a.heat)
+    for (int j = 1; j <= NLEV; ++j) {
+      const R f1 = dftu[j + 1] + dftd[j + 1], f0 = dftu[j] + dftd[j];
+      const R q = a.dq[(long)(j - 1) * NC + c];
+      a.heat[(long)(j - 1) * NC + c] =
+          R(-86400.0 * a.ga) * (f1 - f0) / (R(DS[j]) * ps * R(a.acpd) * (R(1.) + R(a.adv) * q));
+    }
+  if (a.olr) a.olr[c] = -dftu[1];
+}
+
+template <typename R>
+int launch(const exo_lwr_params* p, int ncol, const exo_lwr_columns* in, exo_lwr_out* out,
+           hipStream_t s) {
+  LwrArgs<R> a{};
+  a.nlev = p->nlev;
+  a.ncol = ncol;
+  a.sigma = p->sigma;
+  a.sigmah = p->sigmah;
+  a.dsigma = p->dsigma;
+  a.ga = p->ga;
+  a.th2oc = p->th2oc;
+  a.tpofmt = p->tpofmt;
+  a.acllwr = p->acllwr;
+  a.clgray = p->clgray;
+  a.acpd = p->acpd;
+  a.adv = p->adv;
+  a.npbroaden = p->npbroaden;
+  a.dp = (const R*)in->dp;
+  a.dt = (const R*)in->dt;
+  a.dq = (const R*)in->dq;
+  a.dqo3 = (const R*)in->dqo3;
+  a.dqco2 = (const R*)in->dqco2;
+  a.dcc = (const R*)in->dcc;
+  a.dql = (const R*)in->dql;
+  a.dls = (const R*)in->dls;
+  a.dftu = (R*)out->dftu;
+  a.dftd = (R*)out->dftd;
+  a.heat = (R*)out->heat;
+  a.olr = (R*)out->olr;
+  k_lwr<R><<<(ncol + 127) / 128, 128, 0, s>>>(a);
+  return hipGetLastError() == hipSuccess ? EXO_RAD_OK : EXO_RAD_ERR_HIP;
+}
+
+}  // namespace
+
+extern "C" int exo_lwr(const exo_lwr_params* p, int fp64, int ncol, const exo_lwr_columns* in,
+                       exo_lwr_out* out, void* stream) {
+  if (!p || !in || !out || ncol < 0 || p->nlev < 2 || p->nlev > MAXLEV || !p->sigma ||
+      !p->sigmah || !p->dsigma)
+    return EXO_RAD_ERR_ARG;
+  if (ncol == 0) return EXO_RAD_OK;
+  if (!in->dp || !in->dt || !in->dq || !in->dqo3 || !in->dqco2 || !in->dcc || !in->dql ||
+      !in->dls)
+    return EXO_RAD_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  return fp64 ? launch<double>(p, ncol, in, out, s) : launch<float>(p, ncol, in, out, s);
+}
