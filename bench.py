@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--mode", default="bf16x3")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default tiles instead of timing the variants")
     return ap.parse_args()
 
 
@@ -116,7 +118,11 @@ def main():
     sp_host = np.concatenate([base] * (nmem // 4) + [base[: nmem % 4]])
 
     results = {}
+    tiles = {}
     for mode in (args.mode, "f64"):
+        # plan-time tile choice (untimed, like building the tables)
+        tiles[mode] = (plan.tiles(mode) if args.no_tune
+                       else plan.autotune(mode, nmem, NLEV))
         sp_in, gp, sp_out = plan.alloc_roundtrip(nmem, NLEV, mode)
         sp_in.copy_(torch.from_numpy(sp_host).to(sp_in.dtype))
         es = sp_in.element_size()
@@ -194,7 +200,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": "T42 L20 round trip (64x128 grid, 946 coeffs, 61 prognostic "
                                "field-levels + dmu + u/v)",
-                   "members_per_step": nmem, "mode": args.mode},
+                   "members_per_step": nmem, "mode": args.mode,
+                   "tiles": {m: dict(zip(("inverse", "inverse_uv", "direct"), v))
+                             for m, v in tiles.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp64_gpu": {"value": round(results["f64"]["rt_per_s"], 1),

@@ -17,6 +17,8 @@ struct exo_sht_plan_s {
   void* stage = nullptr;    // device staging of the host drop-in calls
   size_t stage_bytes = 0;
   int nlev = 0;
+  // tile variants per precision mode, chosen by exo_sht_autotune
+  int inv_var[4] = {0, 0, 0, 0}, uv_var[4] = {0, 0, 0, 0}, fwd_var[4] = {0, 0, 0, 0};
   std::mutex mu;
 };
 
@@ -223,6 +225,7 @@ extern "C" int exo_sht_inverse(exo_sht_plan_t p, int mode, int table, int nmem, 
   a.dst0 = dst0;
   a.nfield = nfield;
   a.plavor = p->tb.plavor;
+  a.var = p->inv_var[mode];
   return launch_inverse(p->tb, mode, true, a, (hipStream_t)stream);
 }
 
@@ -244,6 +247,7 @@ extern "C" int exo_sht_inverse_uv(exo_sht_plan_t p, int mode, int nmem, const vo
   a.v0 = v0;
   a.nlev = nlev;
   a.plavor = p->tb.plavor;
+  a.var = p->uv_var[mode];
   return launch_inverse(p->tb, mode, true, a, (hipStream_t)stream);
 }
 
@@ -260,6 +264,7 @@ static int direct_impl(exo_sht_plan_t p, int mode, bool fused, int nmem, const v
   a.dst0 = dst0;
   a.nfield = nfield;
   a.table = TQC;
+  a.var = fused ? p->fwd_var[mode] : 0;
   const size_t need = forward_scratch_bytes(p->tb, mode, nmem, nfield);
   if (need > 0) {
     int rc = grow(&p->scratch, &p->scratch_bytes, need);
@@ -295,6 +300,84 @@ extern "C" int exo_sht_roundtrip(exo_sht_plan_t p, int mode, int nmem, int nlev,
                           nlev, stream);
   if (rc) return rc;
   return exo_sht_direct(p, mode, nmem, gp, gps, 0, sp_out, sps, 0, nfl, stream);
+}
+
+extern "C" int exo_sht_set_tiles(exo_sht_plan_t p, int mode, int inv_var, int uv_var,
+                                 int fwd_var) {
+  if (!p || !mode_ok(mode) || inv_var < 0 || inv_var > 1 || uv_var < 0 || uv_var > 1 ||
+      fwd_var < 0 || fwd_var > 1)
+    return EXO_ERR_ARG;
+  p->inv_var[mode] = inv_var;
+  p->uv_var[mode] = uv_var;
+  p->fwd_var[mode] = fwd_var;
+  return EXO_OK;
+}
+
+extern "C" int exo_sht_get_tiles(exo_sht_plan_t p, int mode, int* vars) {
+  if (!p || !mode_ok(mode) || !vars) return EXO_ERR_ARG;
+  vars[0] = p->inv_var[mode];
+  vars[1] = p->uv_var[mode];
+  vars[2] = p->fwd_var[mode];
+  return EXO_OK;
+}
+
+// time each tile variant of the three fused kernels on a round trip of
+// nmem members x nlev levels and keep the fastest
+extern "C" int exo_sht_autotune(exo_sht_plan_t p, int mode, int nmem, int nlev, void* stream,
+                                int* vars) {
+  if (!p || !mode_ok(mode) || nmem < 1 || nlev < 1) return EXO_ERR_ARG;
+  const Tables& t = p->tb;
+  hipStream_t s = (hipStream_t)stream;
+  const int nfl = 3 * nlev + 1, ngp = nfl + 1 + 2 * nlev;
+  const long sps = (long)nfl * 2 * t.ncsp, gps = (long)ngp * t.nlat * t.nlon;
+  const size_t es = (mode == MODE_F64) ? 8 : 4;
+  void *sp = nullptr, *gp = nullptr, *so = nullptr;
+  HIPCK(hipMalloc(&sp, es * sps * nmem));
+  HIPCK(hipMalloc(&gp, es * gps * nmem));
+  HIPCK(hipMalloc(&so, es * sps * nmem));
+  HIPCK(hipMemsetAsync(sp, 0x3f, es * sps * nmem, s));  // finite, nonzero data
+  hipEvent_t e0, e1;
+  HIPCK(hipEventCreate(&e0));
+  HIPCK(hipEventCreate(&e1));
+  int rc = EXO_OK;
+  float best[3] = {1e30f, 1e30f, 1e30f};
+  int pick[3] = {0, 0, 0};
+  for (int v = 0; v < 2 && rc == EXO_OK; ++v) {
+    p->inv_var[mode] = p->uv_var[mode] = p->fwd_var[mode] = v;
+    for (int k = 0; k < 3 && rc == EXO_OK; ++k) {
+      float ms_min = 1e30f;
+      for (int rep = 0; rep < 4 && rc == EXO_OK; ++rep) {
+        (void)hipEventRecord(e0, s);
+        if (k == 0)
+          rc = exo_sht_inverse(p, mode, EXO_QI, nmem, sp, sps, 0, gp, gps, 0, nfl, s);
+        else if (k == 1)
+          rc = exo_sht_inverse_uv(p, mode, nmem, sp, sps, 0, nlev, gp, gps, nfl + 1,
+                                  nfl + 1 + nlev, nlev, s);
+        else
+          rc = exo_sht_direct(p, mode, nmem, gp, gps, 0, so, sps, 0, nfl, s);
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (rep > 0 && ms < ms_min) ms_min = ms;  // first run warms up
+      }
+      if (ms_min < best[k]) {
+        best[k] = ms_min;
+        pick[k] = v;
+      }
+    }
+  }
+  p->inv_var[mode] = pick[0];
+  p->uv_var[mode] = pick[1];
+  p->fwd_var[mode] = pick[2];
+  if (vars)
+    for (int k = 0; k < 3; ++k) vars[k] = pick[k];
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(sp);
+  (void)hipFree(gp);
+  (void)hipFree(so);
+  return rc;
 }
 
 // ---------------------------------------------------------------------------

@@ -51,6 +51,7 @@ struct InvArgs {
   int src0, dst0, nfield;               // INV_SCALAR field runs
   int vor0, div0, u0, v0, nlev;         // INV_UV
   double plavor;
+  int var;          // tile variant (0 default, 1 half the fields per block)
 };
 
 struct FwdArgs {
@@ -60,6 +61,7 @@ struct FwdArgs {
   int nmem;
   int src0, dst0, nfield;
   int table;        // TQC
+  int var;          // tile variant (0 default, 1 half the fields per block)
 };
 
 // One term of a multi-input direct transform: out[oc] += sign * t * X_in[ic]

@@ -30,20 +30,24 @@ using RealT = typename std::conditional<MODE == MODE_F64, double, float>::type;
 // ---------------------------------------------------------------------------
 // tile shapes
 // ---------------------------------------------------------------------------
-template <int T, int MODE>
+// V selects the tile variant the autotuner times: V = 1 halves the fields
+// per block (half the LDS, twice the blocks in flight).
+template <int T, int MODE, int V = 0>
 struct InvCfg {
   static constexpr int LP = 8;  // latitude pairs per tile (= 16 MFMA rows)
-  static constexpr int FB =
+  static constexpr int FB0 =
       (T == 127) ? (MODE == MODE_F64 ? 2 : 4) : (MODE == MODE_F64 ? 4 : 8);
+  static constexpr int FB = (FB0 >> V) < 2 ? 2 : (FB0 >> V);
 };
 
-template <int T, int MODE>
+template <int T, int MODE, int V = 0>
 struct FwdCfg {
   static constexpr int PB = (T == 21) ? 16
                           : (T == 42) ? 32
                           : (T == 63) ? (MODE == MODE_F64 ? 24 : 48)
                                       : (MODE == MODE_F64 ? 16 : 32);
-  static constexpr int FBF = (T == 21) ? 4 : (T == 42) ? (MODE == MODE_F64 ? 1 : 2) : 1;
+  static constexpr int FBF0 = (T == 21) ? 4 : (T == 42) ? (MODE == MODE_F64 ? 1 : 2) : 1;
+  static constexpr int FBF = (FBF0 >> V) < 1 ? 1 : (FBF0 >> V);
   static constexpr int NPB = TS<T>::NPAIR / PB;
 };
 
@@ -59,13 +63,13 @@ __device__ __forceinline__ f32x4 mfma_bf16(const BF8& a, const BF8& b, f32x4 c) 
 // ---------------------------------------------------------------------------
 // inverse transform
 // ---------------------------------------------------------------------------
-template <int T, int MODE, int KIND, bool FUSED>
+template <int T, int MODE, int KIND, bool FUSED, int V>
 __global__ __launch_bounds__(256) void k_inv(InvArgs a, const void* tabA0, const void* tabA1,
                                              long plane, const void* tabv, const void* twp) {
   using S = TS<T>;
   using R = RealT<MODE>;
   constexpr int NM = S::NM, N = S::NLON, NLAT = S::NLAT, NPB = S::NPAIR / 8;
-  constexpr int LP = InvCfg<T, MODE>::LP, FB = InvCfg<T, MODE>::FB;
+  constexpr int LP = InvCfg<T, MODE, V>::LP, FB = InvCfg<T, MODE, V>::FB;
   constexpr int NF = FftBatch<N>::NF;
   constexpr int NTERM = (KIND == INV_UV) ? 2 : 1;
 
@@ -374,12 +378,12 @@ __global__ __launch_bounds__(256) void k_inv(InvArgs a, const void* tabA0, const
 // hold the Fourier coefficients of only FBF (1-4) fields in LDS, which would
 // fill 2-8 of the 16 columns of an MFMA tile while the operand split would
 // still cost the whole wave (see the design notes).
-template <int T, int MODE, bool FUSED>
+template <int T, int MODE, bool FUSED, int V>
 __global__ __launch_bounds__(256) void k_fwd(FwdArgs a, const void* tabv, const void* twp,
                                              void* scratch) {
   using S = TS<T>;
   using R = RealT<MODE>;
-  using C = FwdCfg<T, MODE>;
+  using C = FwdCfg<T, MODE, V>;
   constexpr int NM = S::NM, N = S::NLON, NLAT = S::NLAT;
   constexpr int PB = C::PB, FBF = C::FBF, NPB = C::NPB;
   constexpr int NCOL = 2 * FBF;
@@ -573,9 +577,9 @@ __global__ __launch_bounds__(256) void k_fft_rows(R* a, long lot, const C2<R>* t
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int T, int MODE>
-static int inv_t(const Tables& tb, bool fused, const InvArgs& a, hipStream_t s) {
-  using C = InvCfg<T, MODE>;
+template <int T, int MODE, int V>
+static int inv_tv(const Tables& tb, bool fused, const InvArgs& a, hipStream_t s) {
+  using C = InvCfg<T, MODE, V>;
   constexpr int NPB = TS<T>::NPAIR / 8;
   const int ngroups = (a.kind == INV_UV) ? (a.nlev + C::FB / 2 - 1) / (C::FB / 2)
                                          : (a.nfield + C::FB - 1) / C::FB;
@@ -595,16 +599,23 @@ static int inv_t(const Tables& tb, bool fused, const InvArgs& a, hipStream_t s) 
   dim3 grid((unsigned)nblk), block(256);
   if (a.kind == INV_UV) {
     if (fused)
-      k_inv<T, MODE, INV_UV, true><<<grid, block, 0, s>>>(a, A0, A1, tb.inv_elems, tv, tw);
+      k_inv<T, MODE, INV_UV, true, V><<<grid, block, 0, s>>>(a, A0, A1, tb.inv_elems, tv, tw);
     else
-      k_inv<T, MODE, INV_UV, false><<<grid, block, 0, s>>>(a, A0, A1, tb.inv_elems, tv, tw);
+      k_inv<T, MODE, INV_UV, false, V><<<grid, block, 0, s>>>(a, A0, A1, tb.inv_elems, tv, tw);
   } else {
     if (fused)
-      k_inv<T, MODE, INV_SCALAR, true><<<grid, block, 0, s>>>(a, A0, A1, tb.inv_elems, tv, tw);
+      k_inv<T, MODE, INV_SCALAR, true, V><<<grid, block, 0, s>>>(a, A0, A1, tb.inv_elems, tv, tw);
     else
-      k_inv<T, MODE, INV_SCALAR, false><<<grid, block, 0, s>>>(a, A0, A1, tb.inv_elems, tv, tw);
+      k_inv<T, MODE, INV_SCALAR, false, V><<<grid, block, 0, s>>>(a, A0, A1, tb.inv_elems, tv, tw);
   }
   return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+// the unfused (drop-in) transforms always use the default tiles
+template <int T, int MODE>
+static int inv_t(const Tables& tb, bool fused, const InvArgs& a, hipStream_t s) {
+  if (fused && a.var == 1) return inv_tv<T, MODE, 1>(tb, fused, a, s);
+  return inv_tv<T, MODE, 0>(tb, fused, a, s);
 }
 
 template <int T, int MODE>
@@ -614,10 +625,10 @@ static size_t fwd_scratch_t(int nmem, int nfield) {
   return (size_t)C::NPB * nmem * nfield * TS<T>::NRSP * sizeof(RealT<MODE>);
 }
 
-template <int T, int MODE>
-static int fwd_t(const Tables& tb, bool fused, const FwdArgs& a, void* scratch, size_t sbytes,
-                 hipStream_t s) {
-  using C = FwdCfg<T, MODE>;
+template <int T, int MODE, int V>
+static int fwd_tv(const Tables& tb, bool fused, const FwdArgs& a, void* scratch, size_t sbytes,
+                  hipStream_t s) {
+  using C = FwdCfg<T, MODE, V>;
   using R = RealT<MODE>;
   const int ngroups = (a.nfield + C::FBF - 1) / C::FBF;
   const long nblk = (long)a.nmem * ngroups * C::NPB;
@@ -628,15 +639,22 @@ static int fwd_t(const Tables& tb, bool fused, const FwdArgs& a, void* scratch, 
   const void* tw = (MODE == MODE_F64) ? (const void*)tb.tw64 : (const void*)tb.tw32;
   dim3 grid((unsigned)nblk), block(256);
   if (fused)
-    k_fwd<T, MODE, true><<<grid, block, 0, s>>>(a, tv, tw, scratch);
+    k_fwd<T, MODE, true, V><<<grid, block, 0, s>>>(a, tv, tw, scratch);
   else
-    k_fwd<T, MODE, false><<<grid, block, 0, s>>>(a, tv, tw, scratch);
+    k_fwd<T, MODE, false, V><<<grid, block, 0, s>>>(a, tv, tw, scratch);
   if (C::NPB > 1) {
     const long total = (long)a.nmem * a.nfield * TS<T>::NRSP;
     k_reduce_partials<R><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
         (const R*)scratch, (R*)a.sp, C::NPB, a.nmem, a.nfield, TS<T>::NRSP, a.sp_stride, a.dst0);
   }
   return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+template <int T, int MODE>
+static int fwd_t(const Tables& tb, bool fused, const FwdArgs& a, void* scratch, size_t sbytes,
+                 hipStream_t s) {
+  if (fused && a.var == 1) return fwd_tv<T, MODE, 1>(tb, fused, a, scratch, sbytes, s);
+  return fwd_tv<T, MODE, 0>(tb, fused, a, scratch, sbytes, s);
 }
 
 #define EXO_DISPATCH_T(fn, ...)                                   \

@@ -295,6 +295,23 @@ class Plan:
                                       _ptr(sp_in), _ptr(gp), _ptr(sp_out), _stream()),
               "roundtrip")
 
+    def set_tiles(self, mode, inv_var=0, uv_var=0, fwd_var=0):
+        check(lib().exo_sht_set_tiles(self._h, mode_id(mode), inv_var, uv_var, fwd_var),
+              "set_tiles")
+
+    def tiles(self, mode):
+        v = (c_int * 3)()
+        check(lib().exo_sht_get_tiles(self._h, mode_id(mode), v), "get_tiles")
+        return tuple(v)
+
+    def autotune(self, mode, nmem: int, nlev: int):
+        """Time the tile variants of the three fused kernels and keep the
+        fastest (returns (inverse, inverse_uv, direct) variants)."""
+        v = (c_int * 3)()
+        check(lib().exo_sht_autotune(self._h, mode_id(mode), nmem, nlev, _stream(), v),
+              "autotune")
+        return tuple(v)
+
     def alloc_roundtrip(self, nmem: int, nlev: int, mode):
         """Zeroed device buffers (sp_in, gp, sp_out) of a round trip."""
         import torch

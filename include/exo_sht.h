@@ -155,6 +155,15 @@ int exo_sht_direct(exo_sht_plan_t plan, int mode, int nmem, const void *gp, long
 int exo_sht_roundtrip(exo_sht_plan_t plan, int mode, int nmem, int nlev, const void *sp_in,
                       void *gp, void *sp_out, void *stream);
 
+/* Tile variants of the three fused kernels (inverse sp2fc/sp2fcdmu,
+ * inverse dv2uv, direct): 0 = default tiles, 1 = half the field-levels per
+ * block (half the LDS, twice the blocks in flight).  exo_sht_autotune times
+ * both on a round trip of nmem members x nlev levels, keeps the fastest of
+ * each kernel for this mode and returns them in vars[3] (may be NULL). */
+int exo_sht_set_tiles(exo_sht_plan_t plan, int mode, int inv_var, int uv_var, int fwd_var);
+int exo_sht_get_tiles(exo_sht_plan_t plan, int mode, int *vars);
+int exo_sht_autotune(exo_sht_plan_t plan, int mode, int nmem, int nlev, void *stream, int *vars);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,14 +78,9 @@ def _oracle_roundtrip(oracle, ntru, nlev, sp):
     return np.stack(gps), np.stack(sps)
 
 
-@pytest.mark.parametrize("mode", ["f64", "f32", "f32mfma", "bf16x3"])
-@pytest.mark.parametrize("ntru,nlev", TRUNCS)
-def test_roundtrip_vs_oracle(plans, oracle, gpu, mode, ntru, nlev):
+def _check_roundtrip(p, oracle, mode, ntru, nlev, nmem=2):
     import torch
 
-    print(f"roundtrip T{ntru} L{nlev} {mode}", flush=True)
-    p = get_plan(plans, ntru)
-    nmem = 2
     sp = ensemble(ntru, nlev, nmem, cfg=2)
     g_ref, s_ref = _oracle_roundtrip(oracle, ntru, nlev, sp)
     sp_in, gp, sp_out = p.alloc_roundtrip(nmem, nlev, mode)
@@ -112,6 +107,43 @@ def test_roundtrip_vs_oracle(plans, oracle, gpu, mode, ntru, nlev):
             assert e < tol, (k, name, "spectral", e)
     # the round trip reproduces its band-limited input
     assert rel_l2(s, sp) < max(tol, 1e-12) * 10
+
+
+@pytest.mark.parametrize("mode", ["f64", "f32", "f32mfma", "bf16x3"])
+@pytest.mark.parametrize("ntru,nlev", TRUNCS)
+def test_roundtrip_vs_oracle(plans, oracle, gpu, mode, ntru, nlev):
+    print(f"roundtrip T{ntru} L{nlev} {mode}", flush=True)
+    p = get_plan(plans, ntru)
+    p.set_tiles(mode, 0, 0, 0)
+    _check_roundtrip(p, oracle, mode, ntru, nlev)
+
+
+@pytest.mark.parametrize("mode", ["f64", "f32", "f32mfma", "bf16x3"])
+@pytest.mark.parametrize("ntru,nlev", [(21, 10), (42, 20), (127, 4)])
+def test_roundtrip_tile_variant(oracle, gpu, mode, ntru, nlev):
+    # the half-field tiles (variant 1) of all three fused kernels; nlev=20
+    # also leaves a partial last block for the odd field counts
+    print(f"roundtrip tiles=1 T{ntru} L{nlev} {mode}", flush=True)
+    p = sht.Plan(ntru)
+    try:
+        p.set_tiles(mode, 1, 1, 1)
+        assert p.tiles(mode) == (1, 1, 1)
+        _check_roundtrip(p, oracle, mode, ntru, nlev, nmem=3)
+    finally:
+        p.close()
+
+
+def test_autotune(oracle, gpu):
+    print("autotune T42 L20 bf16x3", flush=True)
+    p = sht.Plan(42)
+    try:
+        v = p.autotune("bf16x3", 16, 20)
+        assert all(x in (0, 1) for x in v) and p.tiles("bf16x3") == v
+        _check_roundtrip(p, oracle, "bf16x3", 42, 20)
+        with pytest.raises(sht.ExoError):
+            p.set_tiles("bf16x3", 2, 0, 0)
+    finally:
+        p.close()
 
 
 def test_dropin_f64_t21(oracle, gpu):
