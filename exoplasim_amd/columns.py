@@ -38,6 +38,6 @@ def make_columns(ncol: int, nlev: int, seed: int) -> dict:
     q = np.exp(rng.uniform(np.log(1e-7), np.log(2e-2), (ncol, nlev)))
     ps = rng.uniform(0.1, 10.0, ncol) * 1e5
     co2 = np.exp(rng.uniform(0.0, np.log(1e5), ncol))
-    zeros = np.zeros((ncol, nlev))
     return {"dt": dt, "dq": q, "dp": ps, "dqco2": np.repeat(co2[:, None], nlev, axis=1),
-            "dqo3": zeros, "dcc": zeros, "dql": zeros, "dls": np.zeros(ncol)}
+            "dqo3": np.zeros((ncol, nlev)), "dcc": np.zeros((ncol, nlev)),
+            "dql": np.zeros((ncol, nlev)), "dls": np.zeros(ncol)}

@@ -31,32 +31,38 @@ def gelu_d(z):
     return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2 * np.pi)
 
 
-def forward(x, params):
+def exact(x: np.ndarray) -> np.ndarray:
+    """no rounding: the plain FP64 network (for checking the calculus)"""
+    return np.asarray(x, dtype=np.float64)
+
+
+def forward(x, params, rnd=bf16):
     """x (M, 43) float; params [W1, b1, ..., W4, b4] with W (out, in).
-    Returns y (M, 21) and the saved (inputs, pre-activations)."""
-    h = bf16(x)
+    Returns y (M, 21) and the saved (inputs, pre-activations).  rnd is the
+    rounding applied where the kernels hold bf16 values."""
+    h = rnd(x)
     saved = []
     for li in range(4):
         w, b = params[2 * li], params[2 * li + 1]
-        z = h @ bf16(w).T + b
+        z = h @ rnd(w).T + b
         saved.append((h, z))
-        h = bf16(gelu(z)) if li < 3 else z
+        h = rnd(gelu(z)) if li < 3 else z
     return h, saved
 
 
-def backward(gy, params, saved):
+def backward(gy, params, saved, rnd=bf16):
     """gradients of sum(gy * y) w.r.t. W1, b1, ..., W4, b4"""
     grads = [None] * 8
-    dz32 = gy.astype(np.float32).astype(np.float64)
-    dz = bf16(gy)
+    dz32 = gy.astype(np.float32).astype(np.float64) if rnd is bf16 else exact(gy)
+    dz = rnd(gy)
     for li in (3, 2, 1, 0):
         h, z = saved[li]
         grads[2 * li] = dz.T @ h
         grads[2 * li + 1] = dz32.sum(axis=0)
         if li > 0:
-            dh = dz @ bf16(params[2 * li])
+            dh = dz @ rnd(params[2 * li])
             zp = saved[li - 1][1]
             g = dh * gelu_d(zp)
             dz32 = g
-            dz = bf16(g)
+            dz = rnd(g)
     return grads

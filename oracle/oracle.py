@@ -414,8 +414,8 @@ class RefRad:
     columns, NLEV=10), FP64; loaded with lazy binding because radmod's other
     routines reference model subroutines that this library does not link."""
 
-    def __init__(self):
-        self.lib = ctypes.CDLL(_ref_path("libref_rad_t21.so"), mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+    def __init__(self, name: str = "libref_rad_t21.so"):
+        self.lib = ctypes.CDLL(_ref_path(name), mode=os.RTLD_LAZY | os.RTLD_LOCAL)
         d = (c_int * 2)()
         self.lib.ref_rad_dims(d)
         self.nhor, self.nlev = list(d)

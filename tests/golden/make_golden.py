@@ -80,7 +80,50 @@ def main():
     path = os.path.join(HERE, "reference_t21_t42.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes")
+    lwr_golden()
+
+
+# parameter sets of the long-wave fixture: namelist defaults, then grey
+# clouds without pressure broadening
+LWR_PARAMS = [{}, {"clgray": 0.7, "npbroaden": 0, "tpofmt": 0.8, "th2oc": 0.03}]
+LWR_KEEP = 96  # columns stored per parameter set
+
+
+def lwr_columns(ncol, nlev):
+    """synthetic columns covering every branch of lwr: clear and cloudy
+    (random cloud cover and liquid water), ozone, land and ocean"""
+    sys.path.insert(0, ROOT)
+    from exoplasim_amd.columns import make_columns
+
+    c = make_columns(ncol, nlev, seed=77)
+    rng = np.random.default_rng(78)
+    half = ncol // 2
+    c["dcc"][half:] = rng.uniform(0.0, 1.0, (ncol - half, nlev)) * (rng.uniform(size=(ncol - half, nlev)) < 0.6)
+    c["dql"][half:] = rng.uniform(0.0, 3e-4, (ncol - half, nlev)) * (c["dcc"][half:] > 0)
+    c["dqo3"][::3] = rng.uniform(0.0, 1e-5, (len(c["dqo3"][::3]), nlev))
+    c["dls"][1::2] = 1.0
+    return c
+
+
+def lwr_golden():
+    from oracle import RefRad
+
+    r = RefRad()
+    cols = lwr_columns(r.nhor, r.nlev)
+    keep = np.linspace(0, r.nhor - 1, LWR_KEEP).round().astype(int)
+    out = {"keep": keep}
+    for k, v in cols.items():
+        out["in_" + k] = v[keep]
+    for i, prm in enumerate(LWR_PARAMS):
+        fu, fd = r.lwr(cols, **prm)
+        out[f"ftu_{i}"], out[f"ftd_{i}"] = fu[keep], fd[keep]
+    path = os.path.join(HERE, "reference_lwr.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes")
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["lwr"]:
+        lwr_golden()
+    else:
+        main()

@@ -33,7 +33,10 @@ def test_lwr_column_vs_oracle(oracle, gpu):
     fu, fd, heat, olr = O.lwr(oracle, cols, nlev)
     heat_kday = heat * 86400.0
     dcols = columns_to_device(cols)
-    for fp64, tol in ((True, 1e-11), (False, 2e-4)):
+    # fp64: 1e-10, the reference's own -O0/-O2 spread where the effective
+    # radiation cancels near ztau0 = 1 - 1e-6 (see test_oracle.LWR_TOL);
+    # fp32 interface and arithmetic: 2e-4
+    for fp64, tol in ((True, 1e-10), (False, 2e-4)):
         col = LwrColumn(nlev, fp64=fp64)
         h, o, u, d = col(dcols, fluxes=True)
         torch.cuda.synchronize()
