@@ -39,6 +39,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default tiles instead of timing the variants")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the T42 L20 headline (skip ensemble, T127, emulator)")
+    ap.add_argument("--emulator-columns", type=int, default=2_000_000)
+    ap.add_argument("--emulator-epochs", type=int, default=6)
     return ap.parse_args()
 
 
@@ -94,6 +98,172 @@ def cpu_baseline(seconds: float, sp_member):
             "sample": f"{sum(done)} T42 L20 FP64 round trips of one seeded member on "
                       f"{cores} host threads in {el:.1f} s (oracle/exo_oracle.c, "
                       "gcc -O2)"}
+
+
+# ---------------------------------------------------------------------------
+# extra workloads reported beside the headline
+# ---------------------------------------------------------------------------
+ENS_GROUPS = [(21, 10, 48), (42, 20, 32), (63, 20, 16)]
+
+
+def bench_ensemble(args, ora):
+    """96 members of mixed truncation: one graph launch of the two persistent
+    kernels vs one batched round trip per truncation vs one round trip per
+    member."""
+    import numpy as np
+    import torch
+
+    from exoplasim_amd import sht
+    from exoplasim_amd.synth import ensemble
+
+    ens = sht.Ensemble(ENS_GROUPS, args.mode)
+    host = []
+    for k, (ntru, nlev, nmem) in enumerate(ENS_GROUPS):
+        sp = ensemble(ntru, nlev, nmem, cfg=3)
+        ens.bufs[k][0].copy_(torch.from_numpy(sp).float())
+        host.append(sp)
+    tiles = ens.autotune()
+    steps = max(5, args.steps // 2)
+    t_graph = time_loop(ens.run, steps, 3)
+    t_groups = time_loop(ens.run_groups, steps, 3)
+    t_members = time_loop(ens.run_members, max(3, steps // 5), 1)
+    nm = ens.members
+    ens.run()
+    torch.cuda.synchronize()
+    err = 0.0
+    for k, (ntru, nlev, nmem) in enumerate(ENS_GROUPS):
+        g, s_ = ora.roundtrip(ntru, nlev, host[k][0])
+        err = max(err, rel_l2(ens.bufs[k][1][0].double().cpu().numpy(), g),
+                  rel_l2(ens.bufs[k][2][0].double().cpu().numpy(), s_))
+    ens.close()
+    r = lambda t, n: round(nm * n / t, 1)  # noqa: E731
+    return {"groups": [{"ntru": a, "nlev": b, "members": c} for a, b, c in ENS_GROUPS],
+            "mode": args.mode, "tiles": tiles, "unit": "member round trips/s",
+            "graph_persistent": r(t_graph, steps),
+            "batched_per_truncation": r(t_groups, steps),
+            "loop_of_single_members": r(t_members, max(3, steps // 5)),
+            "speedup_vs_member_loop": round(t_members / max(3, steps // 5) / (t_graph / steps), 2),
+            "ms_per_ensemble_graph": round(t_graph / steps * 1e3, 4),
+            "max_rel_l2_vs_fp64_oracle": err}
+
+
+def _mfma_flops_inverse(ntru, nfield, nmem, fb, nterm):
+    """MFMA flops executed by a bf16x3 inverse launch: per block (8 pairs x
+    fb fields) and m, ceil((T+1-m)/32) 16x16x32 tiles, 6 products each"""
+    nm = ntru + 1
+    tiles = sum((nm - m + 31) // 32 for m in range(nm))
+    groups = (nfield + fb - 1) // fb
+    nblk = nmem * groups * ({21: 32, 42: 64, 63: 96, 127: 192}[ntru] // 2 // 8)
+    return nblk * tiles * 6 * nterm * 2 * 16 * 16 * 32
+
+
+def bench_t127(args, ora):
+    """T127 L40 split-BF16 round trips: HBM GB/s, MFMA rate, error."""
+    import numpy as np
+    import torch
+
+    from exoplasim_amd import sht
+    from exoplasim_amd.synth import ensemble
+
+    ntru, nlev, nmem = 127, 40, 8
+    plan = sht.Plan(ntru)
+    mode = args.mode
+    tiles = plan.autotune(mode, nmem, nlev)
+    sp = ensemble(ntru, nlev, nmem, cfg=4, slope=2.0, wave1=10.0)
+    sp_in, gp, sp_out = plan.alloc_roundtrip(nmem, nlev, mode)
+    sp_in.copy_(torch.from_numpy(sp).to(sp_in.dtype))
+    nfl = 3 * nlev + 1
+    steps = max(5, args.steps // 2)
+    t = time_loop(lambda: plan.roundtrip(mode, sp_in, gp, sp_out, nlev), steps, 2) / steps
+    es = sp_in.element_size()
+    nbytes = nmem * (nfl + 1 + 2 * nlev + nfl) * (plan.nrsp + plan.nlat * plan.nlon) * es
+    t_qi = time_loop(lambda: plan.inverse(mode, sp_in, gp, nfl, 0, 0, "qi"), steps, 2) / steps
+    fb = 4 if tiles[0] == 0 else 2
+    fl_qi = _mfma_flops_inverse(ntru, nfl, nmem, fb, 1)
+    torch.cuda.synchronize()
+    plan.roundtrip(mode, sp_in, gp, sp_out, nlev)
+    g, s_ = ora.roundtrip(ntru, nlev, sp[0])
+    torch.cuda.synchronize()
+    eg = rel_l2(gp[0].double().cpu().numpy(), g)
+    es_ = rel_l2(sp_out[0].double().cpu().numpy(), s_)
+    plan.close()
+    return {"members": nmem, "mode": mode, "tiles": tiles,
+            "rt_per_s": round(nmem / t, 1), "ms_per_step": round(t * 1e3, 4),
+            "roundtrip_GBs": round(nbytes / t / 1e9, 1),
+            "inverse_qi_ms": round(t_qi * 1e3, 4),
+            "inverse_qi_mfma_TFLOPs": round(fl_qi / t_qi / 1e12, 1),
+            "inverse_qi_mfma_frac_of_bf16_peak": round(fl_qi / t_qi / 2.5e15, 4),
+            "grid_rel_l2_vs_fp64_oracle": eg, "spectral_rel_l2_vs_fp64_oracle": es_}
+
+
+def bench_emulator(args, ora):
+    """Train the long-wave emulator on columns from the GPU reference column,
+    then time 8192-column steps against the plain column (GPU and CPU)."""
+    import numpy as np
+    import torch
+
+    from exoplasim_amd.columns import make_columns
+    from exoplasim_amd.radiation import LwEmulator, LwrColumn, columns_to_device, features
+
+    nlev, ncol = 20, 8192
+    col = LwrColumn(nlev, fp64=False)
+    em = LwEmulator(seed=0)
+    xs, ys = [], []
+    chunk = 200_000
+    t0 = time.perf_counter()
+    for k in range((args.emulator_columns + chunk - 1) // chunk):
+        c = columns_to_device(make_columns(chunk, nlev, 1000 + k), torch.float32)
+        h, o = col(c)
+        xs.append(features(c, nlev))
+        ys.append(em.targets(h, o, c))
+    x, y = torch.cat(xs)[: args.emulator_columns], torch.cat(ys)[: args.emulator_columns]
+    del xs, ys
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    em.fit(x, y, epochs=args.emulator_epochs, batch=4096, lr=2e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del x, y
+    test_np = make_columns(ncol, nlev, 7)
+    test = columns_to_device(test_np, torch.float32)
+    h_ref, o_ref = col(test)
+    h_em, o_em = em.predict(test)
+    rms = float(torch.sqrt(torch.mean((h_em - h_ref) ** 2)))
+    sig = float(torch.sqrt(torch.mean(h_ref ** 2)))
+    rms_olr = float(torch.sqrt(torch.mean((o_em - o_ref) ** 2)))
+    t_col = time_loop(lambda: col(test), 20, 3) / 20
+    t_em = time_loop(lambda: em.predict(test), 20, 3) / 20
+    # CPU: the oracle column (C restatement of lwr) on host threads
+    import oracle as O
+
+    cores = min(16, os.cpu_count() or 1)
+    sub = {k: v[: 256] for k, v in test_np.items()}
+    done = [0] * cores
+    stop = time.perf_counter() + min(args.cpu_seconds, 5.0)
+
+    def work(i):
+        while time.perf_counter() < stop:
+            O.lwr(ora, sub, nlev)
+            done[i] += 256
+
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(cores)]
+    tc = time.perf_counter()
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    cpu_cols = sum(done) / (time.perf_counter() - tc)
+    return {"train_columns": int(args.emulator_columns), "epochs": args.emulator_epochs,
+            "data_s": round(t_data, 2), "train_s": round(t_train, 2),
+            "heating_rms_K_per_day": round(rms, 4), "heating_signal_rms_K_per_day": round(sig, 3),
+            "olr_rms_W_m2": round(rms_olr, 3),
+            "columns_per_step": ncol,
+            "emulator_columns_per_s": round(ncol / t_em, 1),
+            "gpu_plain_column_columns_per_s": round(ncol / t_col, 1),
+            "cpu_plain_column_columns_per_s": round(cpu_cols, 1), "cpu_cores": cores,
+            "gain_vs_gpu_plain_column": round(t_col / t_em, 2),
+            "gain_vs_cpu_plain_column": round(ncol / t_em / cpu_cols, 1)}
 
 
 def main():
@@ -185,6 +355,17 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_seconds, sp_host[0])
+    extras = {}
+    if not args.no_extras:
+        for name, fn in (("ensemble", bench_ensemble), ("t127", bench_t127),
+                         ("emulator", bench_emulator)):
+            t0 = time.perf_counter()
+            try:
+                extras[name] = fn(args, ora)
+            except Exception as exc:  # reported, never hidden
+                extras[name] = {"error": f"{type(exc).__name__}: {exc}"}
+            print(f"[bench] {name} done in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                  flush=True)
     out = {
         "metric": "T42-L20 gp<->sp round trips/s, split-BF16 vs FP64 GPU at <=1e-6 rel L2",
         "value": round(head["rt_per_s"], 1),
@@ -212,6 +393,7 @@ def main():
         "kernels": {m: {k: {"ms": round(v["ms"], 4), "GBs": round(v["GBs"], 1)}
                         for k, v in r["kernels"].items()} for m, r in results.items()},
         "roundtrip_GBs": round(head["bytes"] * args.steps / head["time"] / 1e9, 1),
+        **extras,
     }
     print(json.dumps(out), flush=True)
     plan.close()

@@ -14,9 +14,10 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libexosht.so")
+OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["sht_tables.hip", "sht_transforms.hip", "sht_direct_terms.hip", "sht_capi.hip",
-           "si_solve.hip", "rad_lwr.hip", "rad_mlp.hip"]
+           "sht_ensemble.hip", "si_solve.hip", "rad_lwr.hip", "rad_mlp.hip"]
 HIPFLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions are kept out of every kernel
@@ -31,16 +32,34 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build_library(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+def build_library(verbose: bool = False, jobs: int = 0) -> str:
+    """Compile every source to an object (in parallel), then link."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    os.makedirs(OBJDIR, exist_ok=True)
+    flags = [f for f in HIPFLAGS if f != "-shared"]
+
+    def compile_one(src):
+        obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc(), *flags, "-c", "-o", obj, os.path.join(CSRC, src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        return obj, subprocess.run(cmd, capture_output=True, text=True)
+
+    jobs = jobs or min(len(SOURCES), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(compile_one, SOURCES))
+    failed = [r for _, r in results if r.returncode != 0]
+    if failed:
+        for r in failed:
+            sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building libexosht.so")
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPFLAGS, "-o", tmp, *srcs]
-    if verbose:
-        print(" ".join(cmd), flush=True)
+    cmd = [hipcc(), *HIPFLAGS, "-o", tmp, *[o for o, _ in results]]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libexosht.so")
+        raise RuntimeError("hipcc failed linking libexosht.so")
     os.replace(tmp, LIB)
     return LIB
 

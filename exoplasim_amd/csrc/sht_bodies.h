@@ -1,0 +1,522 @@
+// sht_bodies.h -- the work items of the inverse and direct transforms as
+// device functions, shared by the per-truncation kernels
+// (sht_transforms.hip) and the persistent mixed-truncation ensemble kernels
+// (sht_ensemble.hip).  Each body takes its block index and an LDS pointer,
+// so one kernel can run items of different truncations in turn.
+#pragma once
+#include <type_traits>
+
+#include "sht_internal.h"
+
+namespace exo {
+
+template <int MODE>
+using RealT = typename std::conditional<MODE == MODE_F64, double, float>::type;
+
+// ---------------------------------------------------------------------------
+// tile shapes
+// ---------------------------------------------------------------------------
+// V selects the tile variant the autotuner times: V = 1 halves the fields
+// per block (half the LDS, twice the blocks in flight).
+template <int T, int MODE, int V = 0>
+struct InvCfg {
+  static constexpr int LP = 8;  // latitude pairs per tile (= 16 MFMA rows)
+  static constexpr int FB0 =
+      (T == 127) ? (MODE == MODE_F64 ? 2 : 4) : (MODE == MODE_F64 ? 4 : 8);
+  static constexpr int FB = (FB0 >> V) < 2 ? 2 : (FB0 >> V);
+};
+
+template <int T, int MODE, int V = 0>
+struct FwdCfg {
+  static constexpr int PB = (T == 21) ? 16
+                          : (T == 42) ? 32
+                          : (T == 63) ? (MODE == MODE_F64 ? 24 : 48)
+                                      : (MODE == MODE_F64 ? 16 : 32);
+  static constexpr int FBF0 = (T == 21) ? 4 : (T == 42) ? (MODE == MODE_F64 ? 1 : 2) : 1;
+  static constexpr int FBF = (FBF0 >> V) < 1 ? 1 : (FBF0 >> V);
+  static constexpr int NPB = TS<T>::NPAIR / PB;
+};
+
+template <typename R>
+struct NS2 {
+  C2<R> n, s;  // Fourier coefficient at the northern / southern latitude
+};
+
+__device__ __forceinline__ f32x4 mfma_bf16(const BF8& a, const BF8& b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// inverse transform
+// ---------------------------------------------------------------------------
+// LDS of one inverse block: Fourier coefficients of LP pairs x FB fields,
+// plus the FFT work rows and twiddles when fused
+template <int T, int MODE, bool FUSED, int V>
+struct InvLds {
+  using R = RealT<MODE>;
+  using C = InvCfg<T, MODE, V>;
+  static constexpr size_t FCL = sizeof(NS2<R>) * C::LP * C::FB * TS<T>::NM;
+  static constexpr size_t FBUF = FUSED ? sizeof(C2<R>) * FftBatch<TS<T>::NLON>::NF * TS<T>::NLON : 0;
+  static constexpr size_t TWS = FUSED ? sizeof(C2<R>) * TS<T>::NLON : 0;
+  static constexpr size_t BYTES = FCL + FBUF + TWS;
+};
+
+// number of blocks (work items) of one inverse launch
+template <int T, int MODE, int V>
+__host__ __device__ inline long inv_items(const InvArgs& a) {
+  using C = InvCfg<T, MODE, V>;
+  const int ngroups = (a.kind == INV_UV) ? (a.nlev + C::FB / 2 - 1) / (C::FB / 2)
+                                         : (a.nfield + C::FB - 1) / C::FB;
+  return (long)a.nmem * ngroups * (TS<T>::NPAIR / C::LP);
+}
+
+// One work item of the inverse transform: pair block pb, field group g,
+// member mem, decoded from blk.  lds holds InvLds::BYTES bytes.
+template <int T, int MODE, int KIND, bool FUSED, int V>
+__device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* tabA0,
+                                         const void* tabA1, long plane, const void* tabv,
+                                         const void* twp, char* lds) {
+  using S = TS<T>;
+  using R = RealT<MODE>;
+  constexpr int NM = S::NM, N = S::NLON, NLAT = S::NLAT, NPB = S::NPAIR / 8;
+  constexpr int LP = InvCfg<T, MODE, V>::LP, FB = InvCfg<T, MODE, V>::FB;
+  constexpr int NF = FftBatch<N>::NF;
+  constexpr int NTERM = (KIND == INV_UV) ? 2 : 1;
+
+  using L = InvLds<T, MODE, FUSED, V>;
+  NS2<R>* fcl = (NS2<R>*)lds;
+  C2<R>* fbuf = (C2<R>*)(lds + L::FCL);
+  C2<R>* tws = (C2<R>*)(lds + L::FCL + L::FBUF);
+
+  const int tid = item_tid();
+  const int ngroups = (KIND == INV_UV) ? (a.nlev + FB / 2 - 1) / (FB / 2) : (a.nfield + FB - 1) / FB;
+  const int pb = blk % NPB;
+  const int g = (blk / NPB) % ngroups;
+  const int mem = blk / (NPB * ngroups);
+  if (mem >= a.nmem) return;
+  const R* spm = (const R*)a.sp + mem * a.sp_stride;
+  R* outm = (R*)a.out + mem * a.out_stride;
+
+  if (FUSED) {
+    const C2<R>* tw = (const C2<R>*)twp;
+    for (int k = tid; k < N; k += 256) tws[k] = tw[k];
+  }
+
+  // value of MFMA/GEMM column c (0..2FB-1) for spectral index w, term t
+  auto bval = [&](int term, int c, int w) -> R {
+    const int fl = c >> 1, ri = c & 1;
+    if (KIND == INV_SCALAR) {
+      const int field = g * FB + fl;
+      if (fl >= FB || field >= a.nfield) return R(0);
+      return spm[(long)(a.src0 + field) * S::NRSP + 2 * w + ri];
+    } else {
+      const int lvl = g * (FB / 2) + (fl >> 1);
+      const int isv = fl & 1;
+      if (fl >= FB || lvl >= a.nlev) return R(0);
+      const R* zeta = spm + (long)(a.vor0 + lvl) * S::NRSP;
+      const R* div = spm + (long)(a.div0 + lvl) * S::NRSP;
+      // term 0 (qv): u <- zeta, v <- -D ; term 1 (qu): u <- -iD, v <- -i zeta
+      if (term == 0) {
+        if (!isv) {
+          R z = zeta[2 * w + ri];
+          if (w == 1 && ri == 0) z = z - (R)a.plavor;
+          return z;
+        }
+        return -div[2 * w + ri];
+      } else {
+        const R* x = isv ? zeta : div;
+        if (ri == 0) return x[2 * w + 1];
+        R xr = x[2 * w];
+        if (isv && w == 1) xr = xr - (R)a.plavor;
+        return -xr;
+      }
+    }
+  };
+
+  // ---- phase 1: Legendre sums into fcl ----
+  if constexpr (MODE == MODE_BF16X3 || MODE == MODE_F32MFMA) {
+    // GEMM per m: C[row = 2*pair + class][col] = sum_n A[row][n] B[n][col];
+    // wave wv takes m = wv, wv+4, ...; each (m, 32-wide n block) is one
+    // work item, and the operands of the next item are fetched before the
+    // current one is multiplied.
+    const int lane = tid & 63, wv = tid >> 6;
+    const int q = lane >> 4, c = lane & 15;
+    // column c of B: SCALAR field g*FB + c/2; UV level g*FB/2 + c/4 of
+    // vorticity ((c/2)&1 == 0) or divergence; component c&1
+    const R* bp = nullptr;
+    bool isvor = false;
+    if (KIND == INV_SCALAR) {
+      const int field = g * FB + (c >> 1);
+      if ((c >> 1) < FB && field < a.nfield) bp = spm + (long)(a.src0 + field) * S::NRSP + (c & 1);
+    } else {
+      const int lvl = g * (FB / 2) + (c >> 2);
+      const int s = (c >> 1) & 1;
+      if ((c >> 2) < FB / 2 && lvl < a.nlev) {
+        bp = spm + (long)((s ? a.div0 : a.vor0) + lvl) * S::NRSP + (c & 1);
+        isvor = (s == 0) && ((c & 1) == 0);
+      }
+    }
+    const R pv = (R)a.plavor;
+    // n offset of B element j (bf16: 8 consecutive n per lane; f32: n = 4j + q)
+    auto noff = [&](int j) { return MODE == MODE_BF16X3 ? q * 8 + j : j * 4 + q; };
+    auto fetch = [&](int m, int kb, float* br, uint4* a0, uint4* a1) {
+      const int w0 = w0_of(T, m);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int n = m + kb * 32 + noff(j);
+        R x = R(0);
+        if (bp && n <= T) {
+          x = bp[2 * (w0 + n - m)];
+          if (isvor && m == 0 && n == 1) x = x - pv;
+        }
+        br[j] = (float)x;
+      }
+      const long blk = ((long)(pb * NM + m) * S::KB + kb) * 512;
+      if constexpr (MODE == MODE_BF16X3) {
+        const uint16_t* A0 = (const uint16_t*)tabA0 + blk + c * 32 + q * 8;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) a0[p] = *(const uint4*)(A0 + p * plane);
+        if (NTERM == 2) {
+          const uint16_t* A1 = (const uint16_t*)tabA1 + blk + c * 32 + q * 8;
+#pragma unroll
+          for (int p = 0; p < 3; ++p) a1[p] = *(const uint4*)(A1 + p * plane);
+        }
+      } else {
+        // f32 operand stored as [q][row][8] so one lane reads 8 floats
+        const float* A0 = (const float*)tabA0 + blk + (q * 16 + c) * 8;
+        a0[0] = *(const uint4*)A0;
+        a0[1] = *(const uint4*)(A0 + 4);
+        if (NTERM == 2) {
+          const float* A1 = (const float*)tabA1 + blk + (q * 16 + c) * 8;
+          a1[0] = *(const uint4*)A1;
+          a1[1] = *(const uint4*)(A1 + 4);
+        }
+      }
+    };
+    auto mul = [&](const float* br, const uint4* a, f32x4 acc) -> f32x4 {
+      if constexpr (MODE == MODE_BF16X3) {
+        BF8 bh, bm, bl, ah, am, al;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) split3(br[j], bh.s[j], bm.s[j], bl.s[j]);
+        ah.q = a[0];
+        am.q = a[1];
+        al.q = a[2];
+        acc = mfma_bf16(ah, bl, acc);
+        acc = mfma_bf16(am, bm, acc);
+        acc = mfma_bf16(al, bh, acc);
+        acc = mfma_bf16(ah, bm, acc);
+        acc = mfma_bf16(am, bh, acc);
+        acc = mfma_bf16(ah, bh, acc);
+      } else {
+        const float* af = (const float*)a;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[j], br[j], acc, 0, 0, 0);
+      }
+      return acc;
+    };
+    float br[8], br2[8];
+    uint4 a0[3], a1[3], a02[3], a12[3];
+    int m = wv, kb = 0;
+    if (m <= T) fetch(m, 0, br, a0, a1);
+    f32x4 acc1 = {0.f, 0.f, 0.f, 0.f}, acc2 = {0.f, 0.f, 0.f, 0.f};
+    while (m <= T) {
+      const int kbn = (NM - m + 31) / 32;
+      int m2 = m, kb2 = kb + 1;
+      if (kb2 >= kbn) {
+        m2 = m + 4;
+        kb2 = 0;
+      }
+      if (m2 <= T) fetch(m2, kb2, br2, a02, a12);
+      acc1 = mul(br, a0, acc1);
+      if (NTERM == 2) acc2 = mul(br, a1, acc2);
+      if (kb2 == 0) {
+        // all n of this m are summed: combine the terms, fold N/S, store
+        f32x4 out = acc1;
+        if (KIND == INV_UV) {
+          // u = qv.zeta + qu.(-i D), v = qu.(-i zeta) - qv.D: the qu product of
+          // the partner column (zeta.re <-> D.im, zeta.im <-> D.re)
+          const int plane_ = (lane & ~3) | (3 - (lane & 3));
+          f32x4 p2;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) p2[i] = __shfl(acc2[i], plane_, 64);
+          const int r4 = c & 3;
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            out[i] = (r4 == 0) ? acc1[i] + p2[i]
+                   : (r4 == 1) ? acc1[i] - p2[i]
+                   : (r4 == 2) ? p2[i] - acc1[i]
+                               : -p2[i] - acc1[i];
+        }
+        const int fl = c >> 1, ri = c & 1;
+        if (fl < FB) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int pl = 2 * q + h;
+            const float sy = out[2 * h], an = out[2 * h + 1];
+            NS2<R>& e = fcl[(pl * FB + fl) * NM + m];
+            if (ri == 0) {
+              e.n.x = sy + an;
+              e.s.x = sy - an;
+            } else {
+              e.n.y = sy + an;
+              e.s.y = sy - an;
+            }
+          }
+        }
+        acc1 = f32x4{0.f, 0.f, 0.f, 0.f};
+        acc2 = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+      m = m2;
+      kb = kb2;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) br[j] = br2[j];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        a0[p] = a02[p];
+        a1[p] = a12[p];
+      }
+    }
+  } else {
+    // vector-ALU Legendre: one (pair, column, m) sum per thread
+    const R* tv = (const R*)tabv;
+    const long tsz = (long)S::NPAIR * S::NCSP;
+    constexpr int NPC = LP * 2 * FB;
+    const int t0 = (KIND == INV_UV) ? TQV : a.table;
+    const int t1 = TQU;
+    for (int idx = tid; idx < NPC * NM; idx += 256) {
+      const int pc = idx % NPC, m = idx / NPC;
+      const int pl = pc % LP, c = pc / LP;
+      const int pair = pb * LP + pl;
+      const int w0 = w0_of(T, m);
+      R sy = 0, an = 0;
+      for (int n = m; n <= T; ++n) {
+        const int w = w0 + n - m;
+        const bool even = ((n - m) & 1) == 0;
+#pragma unroll
+        for (int term = 0; term < NTERM; ++term) {
+          const int t = term == 0 ? t0 : t1;
+          const R p = tv[t * tsz + (long)pair * S::NCSP + w];
+          const R x = bval(term, c, w);
+          if (even != table_is_q(t))
+            sy += p * x;
+          else
+            an += p * x;
+        }
+      }
+      const int fl = c >> 1, ri = c & 1;
+      NS2<R>& e = fcl[(pl * FB + fl) * NM + m];
+      if (ri == 0) {
+        e.n.x = sy + an;
+        e.s.x = sy - an;
+      } else {
+        e.n.y = sy + an;
+        e.s.y = sy - an;
+      }
+    }
+  }
+  __syncthreads();
+
+  // destination field of FFT-field fl (-1 if not present)
+  auto dst_of = [&](int fl) -> int {
+    if (KIND == INV_SCALAR) {
+      const int field = g * FB + fl;
+      return field < a.nfield ? a.dst0 + field : -1;
+    }
+    const int lvl = g * (FB / 2) + (fl >> 1);
+    if (lvl >= a.nlev) return -1;
+    return ((fl & 1) ? a.v0 : a.u0) + lvl;
+  };
+
+  if constexpr (!FUSED) {
+    // Fourier output fc(2, NLON/2, NLAT), zero above the truncation
+    for (int idx = tid; idx < LP * FB * (N / 2); idx += 256) {
+      const int m = idx % (N / 2), pf = idx / (N / 2);
+      const int fl = pf % FB, pl = pf / FB;
+      const int d = dst_of(fl);
+      if (d < 0) continue;
+      const int lat = pb * LP + pl;
+      C2<R> vn = {0, 0}, vs = {0, 0};
+      if (m <= T) {
+        vn = fcl[(pl * FB + fl) * NM + m].n;
+        vs = fcl[(pl * FB + fl) * NM + m].s;
+      }
+      R* o = outm + (long)d * NLAT * N;
+      o[(long)lat * N + 2 * m] = vn.x;
+      o[(long)lat * N + 2 * m + 1] = vn.y;
+      o[(long)(NLAT - 1 - lat) * N + 2 * m] = vs.x;
+      o[(long)(NLAT - 1 - lat) * N + 2 * m + 1] = vs.y;
+    }
+  } else {
+    // ---- phase 2: inverse FFT of z = fc_N + i fc_S, Re -> north row,
+    // Im -> south row (imag of m=0 ignored, as fc2gp does) ----
+    for (int f0 = 0; f0 < LP * FB; f0 += NF) {
+      const int nf = min(NF, LP * FB - f0);
+      auto load = [&](int f, int k) -> C2<R> {
+        const int gi = f0 + f, pl = gi / FB, fl = gi % FB;
+        const NS2<R>* e = fcl + (pl * FB + fl) * NM;
+        if (k == 0) return C2<R>{e[0].n.x, e[0].s.x};
+        if (k <= T) {
+          const NS2<R> v = e[k];
+          return C2<R>{v.n.x - v.s.y, v.n.y + v.s.x};
+        }
+        if (k >= N - T) {
+          const NS2<R> v = e[N - k];
+          return C2<R>{v.n.x + v.s.y, v.s.x - v.n.y};
+        }
+        return C2<R>{R(0), R(0)};
+      };
+      auto store = [&](int f, int k, C2<R> v) {
+        const int gi = f0 + f, pl = gi / FB, fl = gi % FB;
+        const int d = dst_of(fl);
+        if (d < 0) return;
+        const int lat = pb * LP + pl;
+        R* o = outm + (long)d * NLAT * N;
+        o[(long)lat * N + k] = v.x;
+        o[(long)(NLAT - 1 - lat) * N + k] = v.y;
+      };
+      fft_run<N, +1, R>(fbuf, tws, nf, load, store);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// direct transform
+// ---------------------------------------------------------------------------
+// One block transforms FBF field-levels of one member over PB latitude pairs
+// (all pairs when NPB == 1, else partial sums reduced by k_reduce_partials).
+// The direct Legendre sum runs on the vector ALUs in every mode: a block can
+// hold the Fourier coefficients of only FBF (1-4) fields in LDS, which would
+// fill 2-8 of the 16 columns of an MFMA tile while the operand split would
+// still cost the whole wave (see the design notes).
+template <int T, int MODE, bool FUSED, int V>
+struct FwdLds {
+  using R = RealT<MODE>;
+  using C = FwdCfg<T, MODE, V>;
+  static constexpr size_t F = sizeof(R) * TS<T>::NM * 2 * C::PB * 2 * C::FBF;
+  static constexpr size_t FBUF = FUSED ? sizeof(C2<R>) * FftBatch<TS<T>::NLON>::NF * TS<T>::NLON : 0;
+  static constexpr size_t TWS = FUSED ? sizeof(C2<R>) * TS<T>::NLON : 0;
+  static constexpr size_t MW = (TS<T>::NCSP + 15) / 16 * 16;
+  static constexpr size_t BYTES = F + FBUF + TWS + MW;
+};
+
+template <int T, int MODE, int V>
+__host__ __device__ inline long fwd_items(const FwdArgs& a) {
+  using C = FwdCfg<T, MODE, V>;
+  return (long)a.nmem * ((a.nfield + C::FBF - 1) / C::FBF) * C::NPB;
+}
+
+template <int T, int MODE, bool FUSED, int V>
+__device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* tabv,
+                                         const void* twp, void* scratch, char* lds) {
+  using S = TS<T>;
+  using R = RealT<MODE>;
+  using C = FwdCfg<T, MODE, V>;
+  constexpr int NM = S::NM, N = S::NLON, NLAT = S::NLAT;
+  constexpr int PB = C::PB, FBF = C::FBF, NPB = C::NPB;
+  constexpr int NCOL = 2 * FBF;
+  constexpr int NF = FftBatch<N>::NF;
+
+  // F[m][cls][pair][col]: cls 0 = fc_N + fc_S (E), cls 1 = fc_N - fc_S (O)
+  using L = FwdLds<T, MODE, FUSED, V>;
+  R* F = (R*)lds;
+  C2<R>* fbuf = (C2<R>*)(lds + L::F);
+  C2<R>* tws = (C2<R>*)(lds + L::F + L::FBUF);
+  uint8_t* mw = (uint8_t*)(lds + L::F + L::FBUF + L::TWS);
+
+  const int tid = item_tid();
+  const int ngroups = (a.nfield + FBF - 1) / FBF;
+  const int pb = blk % NPB;
+  const int g = (blk / NPB) % ngroups;
+  const int mem = blk / (NPB * ngroups);
+  if (mem >= a.nmem) return;
+  const R* inm = (const R*)a.in + mem * a.in_stride;
+
+  for (int m = tid; m <= T; m += 256)
+    for (int n = m; n <= T; ++n) mw[w0_of(T, m) + n - m] = (uint8_t)m;
+  if (FUSED) {
+    const C2<R>* tw = (const C2<R>*)twp;
+    for (int k = tid; k < N; k += 256) tws[k] = C2<R>{tw[k].x, -tw[k].y};
+  }
+  __syncthreads();
+
+  auto fidx = [](int m, int cls, int pl, int col) { return ((m * 2 + cls) * PB + pl) * NCOL + col; };
+
+  if constexpr (FUSED) {
+    for (int f0 = 0; f0 < PB * FBF; f0 += NF) {
+      const int nf = min(NF, PB * FBF - f0);
+      auto load = [&](int f, int k) -> C2<R> {
+        const int gi = f0 + f, pl = gi / FBF, fl = gi % FBF;
+        const int field = g * FBF + fl;
+        if (field >= a.nfield) return C2<R>{R(0), R(0)};
+        const int lat = pb * PB + pl;
+        const R* src = inm + (long)(a.src0 + field) * NLAT * N;
+        return C2<R>{src[(long)lat * N + k], src[(long)(NLAT - 1 - lat) * N + k]};
+      };
+      auto store = [&](int f, int k, C2<R> v) { fbuf[f * N + k] = v; };
+      fft_run<N, -1, R>(fbuf, tws, nf, load, store);
+      // separate the two real rows, scale by 1/N, fold into E/O
+      const R sc = R(0.5) / R(N);
+      for (int idx = tid; idx < nf * NM; idx += 256) {
+        const int f = idx % nf, k = idx / nf;
+        const C2<R> zk = fbuf[f * N + k];
+        const C2<R> zn = fbuf[f * N + ((N - k) % N)];
+        const R cnx = (zk.x + zn.x) * sc, cny = (zk.y - zn.y) * sc;
+        const R csx = (zk.y + zn.y) * sc, csy = (zn.x - zk.x) * sc;
+        const int gi = f0 + f, pl = gi / FBF, fl = gi % FBF;
+        F[fidx(k, 0, pl, 2 * fl)] = cnx + csx;
+        F[fidx(k, 0, pl, 2 * fl + 1)] = cny + csy;
+        F[fidx(k, 1, pl, 2 * fl)] = cnx - csx;
+        F[fidx(k, 1, pl, 2 * fl + 1)] = cny - csy;
+      }
+      __syncthreads();
+    }
+  } else {
+    for (int idx = tid; idx < PB * FBF * NM; idx += 256) {
+      const int pf = idx % (PB * FBF), k = idx / (PB * FBF);
+      const int fl = pf % FBF, pl = pf / FBF;
+      const int field = g * FBF + fl;
+      if (field >= a.nfield) continue;
+      const int lat = pb * PB + pl;
+      const R* src = inm + (long)(a.src0 + field) * NLAT * N;
+      const R nx = src[(long)lat * N + 2 * k], ny = src[(long)lat * N + 2 * k + 1];
+      const R sx = src[(long)(NLAT - 1 - lat) * N + 2 * k];
+      const R sy = src[(long)(NLAT - 1 - lat) * N + 2 * k + 1];
+      F[fidx(k, 0, pl, 2 * fl)] = nx + sx;
+      F[fidx(k, 0, pl, 2 * fl + 1)] = ny + sy;
+      F[fidx(k, 1, pl, 2 * fl)] = nx - sx;
+      F[fidx(k, 1, pl, 2 * fl + 1)] = ny - sy;
+    }
+    __syncthreads();
+  }
+
+  R* outp;
+  if (NPB == 1)
+    outp = (R*)a.sp + mem * a.sp_stride + (long)a.dst0 * S::NRSP;
+  else
+    outp = (R*)scratch + ((long)pb * a.nmem + mem) * a.nfield * S::NRSP;
+  const bool qtab = table_is_q(a.table);
+  const R* tv = (const R*)tabv + (long)a.table * S::NPAIR * S::NCSP + (long)pb * PB * S::NCSP;
+  for (int w = tid; w < S::NCSP; w += 256) {
+    const int m = mw[w];
+    const int n = m + (w - w0_of(T, m));
+    const int cls = ((((n - m) & 1) == 0) != qtab) ? 0 : 1;
+    const R* fr = &F[fidx(m, cls, 0, 0)];
+    R acc[NCOL];
+#pragma unroll
+    for (int c = 0; c < NCOL; ++c) acc[c] = R(0);
+#pragma unroll 4
+    for (int pl = 0; pl < PB; ++pl) {
+      const R p = tv[(long)pl * S::NCSP + w];
+#pragma unroll
+      for (int c = 0; c < NCOL; ++c) acc[c] += p * fr[pl * NCOL + c];
+    }
+#pragma unroll
+    for (int c = 0; c < NCOL; ++c) {
+      const int field = g * FBF + (c >> 1);
+      if (field < a.nfield) outp[(long)field * S::NRSP + 2 * w + (c & 1)] = acc[c];
+    }
+  }
+}
+
+}  // namespace exo

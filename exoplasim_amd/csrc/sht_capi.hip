@@ -27,6 +27,8 @@ struct exo_sht_plan_s {
     if ((x) != hipSuccess) return EXO_ERR_HIP; \
   } while (0)
 
+const Tables& exo::plan_tables(const exo_sht_plan_s* p) { return p->tb; }
+
 static int nlat_of(int ntru) {
   switch (ntru) {
     case 21: return 32;

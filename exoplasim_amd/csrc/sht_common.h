@@ -145,13 +145,24 @@ struct FftBatch {
 // through `store(f, k, v)`; the other passes go through `buf` (NF*N
 // complex).  All 256 threads of the block must call it.  tw[k] =
 // e^{S 2 pi i k/N}.
+
+// The thread index as an opaque value.  A persistent kernel runs many work
+// items of different shapes in one loop; without this the compiler hoists
+// each item type's thread-index arithmetic out of that loop and keeps all of
+// it in registers at once.
+__device__ __forceinline__ int item_tid() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 template <int N, int R, int NS, int S, bool FIRST, bool LAST, typename F,
           typename Load, typename Store>
 __device__ __forceinline__ void fft_pass(C2<F>* buf, const C2<F>* tw, int nf,
                                          Load& load, Store& store) {
   constexpr int NB = N / R;  // butterflies per FFT
   constexpr int BPT = (FftBatch<N>::NF * NB + 255) / 256;
-  const int tid = threadIdx.x;
+  const int tid = item_tid();
   const int total = nf * NB;
   C2<F> v[BPT][R];
 #pragma unroll

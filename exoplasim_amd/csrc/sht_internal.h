@@ -5,6 +5,8 @@
 
 #include "sht_common.h"
 
+struct exo_sht_plan_s;
+
 namespace exo {
 
 __global__ void k_inigau(int klat, double* pz0, double* pzw);
@@ -85,5 +87,8 @@ int launch_forward(const Tables& T, int mode, bool fused, const FwdArgs& a, void
                    size_t scratch_bytes, hipStream_t s);
 size_t forward_scratch_bytes(const Tables& T, int mode, int nmem, int nfield);
 int launch_fft(const Tables& T, int mode, int dir, void* a, int n, long lot, hipStream_t s);
+
+// the device tables of a plan (defined with the plan object)
+const Tables& plan_tables(const exo_sht_plan_s* p);
 
 }  // namespace exo

@@ -324,3 +324,84 @@ class Plan:
         gp = torch.zeros((nmem, ngp, self.nlat, self.nlon), dtype=dt, device=dev)
         sp_out = torch.zeros((nmem, nfl, self.nrsp), dtype=dt, device=dev)
         return sp_in, gp, sp_out
+
+
+class _EnsGroup(ctypes.Structure):
+    _fields_ = [("ntru", c_int), ("nlev", c_int), ("nmem", c_int), ("sp_in", c_void_p),
+                ("gp", c_void_p), ("sp_out", c_void_p)]
+
+
+class Ensemble:
+    """Round trips of an ensemble of members with different truncations.
+
+    groups: list of (ntru, nlev, nmem) with ntru in 21, 42, 63.  The device
+    buffers of each group are allocated here in the layout of
+    Plan.roundtrip (``self.bufs[k] = (sp_in, gp, sp_out)``); all round trips
+    run as two persistent kernels captured in one hipGraph
+    (include/exo_sht.h, exo_sht_ens_*).  mode: "f32", "f32mfma" or "bf16x3".
+    """
+
+    def __init__(self, groups, mode="bf16x3"):
+        import torch
+
+        self.mode = mode
+        self.groups = [tuple(int(x) for x in g) for g in groups]
+        self.bufs = []
+        arr = (_EnsGroup * len(self.groups))()
+        dev = torch.device("cuda")
+        for k, (ntru, nlev, nmem) in enumerate(self.groups):
+            nlat, nlon, ncsp = dims(ntru)
+            nfl = 3 * nlev + 1
+            ngp = nfl + 1 + 2 * nlev
+            b = (torch.zeros((nmem, nfl, 2 * ncsp), dtype=torch.float32, device=dev),
+                 torch.zeros((nmem, ngp, nlat, nlon), dtype=torch.float32, device=dev),
+                 torch.zeros((nmem, nfl, 2 * ncsp), dtype=torch.float32, device=dev))
+            self.bufs.append(b)
+            arr[k] = _EnsGroup(ntru, nlev, nmem, _ptr(b[0]), _ptr(b[1]), _ptr(b[2]))
+        torch.cuda.synchronize()
+        h = c_void_p()
+        check(lib().exo_sht_ens_create(byref(h), mode_id(mode), len(self.groups), arr),
+              "ens_create")
+        self._h = h
+
+    @property
+    def members(self) -> int:
+        return sum(g[2] for g in self.groups)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().exo_sht_ens_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self):
+        """One launch of the captured graph (all round trips)."""
+        check(lib().exo_sht_ens_run(self._h, _stream()), "ens_run")
+
+    def run_eager(self):
+        check(lib().exo_sht_ens_run_eager(self._h, _stream()), "ens_run_eager")
+
+    def run_groups(self):
+        check(lib().exo_sht_ens_run_groups(self._h, _stream()), "ens_run_groups")
+
+    def run_members(self):
+        check(lib().exo_sht_ens_run_members(self._h, _stream()), "ens_run_members")
+
+    def tiles(self):
+        v = (c_int * (2 * len(self.groups)))()
+        check(lib().exo_sht_ens_get_tiles(self._h, v), "ens_get_tiles")
+        return tuple(v)
+
+    def set_tiles(self, vars):
+        v = (c_int * (2 * len(self.groups)))(*vars)
+        check(lib().exo_sht_ens_set_tiles(self._h, v), "ens_set_tiles")
+
+    def autotune(self):
+        v = (c_int * (2 * len(self.groups)))()
+        check(lib().exo_sht_ens_autotune(self._h, _stream(), v), "ens_autotune")
+        return tuple(v)

@@ -8,7 +8,7 @@
  *    take host arrays in the reference layout, run on the GPU and return when
  *    the result is back in the caller's arrays.  The _f64 / _f32 suffix is
  *    the caller's `real` kind (MOST_PREC 8 or 4).  The Fortran ISO_C_BINDING
- *    shim (exoplasim_amd/fortran/exo_legmod_shim.f90) exports the original
+ *    shim (exoplasim_amd/fortran/exo_legmod_capi.f90 and exo_transforms.f90) exports the original
  *    subroutine names on top of them.
  *
  * 2. A device-resident batched API on plans: many independent members
@@ -163,6 +163,43 @@ int exo_sht_roundtrip(exo_sht_plan_t plan, int mode, int nmem, int nlev, const v
 int exo_sht_set_tiles(exo_sht_plan_t plan, int mode, int inv_var, int uv_var, int fwd_var);
 int exo_sht_get_tiles(exo_sht_plan_t plan, int mode, int *vars);
 int exo_sht_autotune(exo_sht_plan_t plan, int mode, int nmem, int nlev, void *stream, int *vars);
+
+/* ====================================================================== */
+/* 3. mixed-truncation ensembles                                           */
+/* ====================================================================== */
+/* No counterpart in the reference, which runs one model per process: an
+ * ensemble of members with different truncations, each group in the layout
+ * of exo_sht_roundtrip (sp_in and sp_out (nmem, 3*nlev+1, 2*NCSP), gp
+ * (nmem, 4*nlev+2, NLAT, NLON), float device arrays).  Truncations 21, 42
+ * and 63; modes EXO_F32, EXO_F32MFMA and EXO_BF16X3.  The round trips of all
+ * members run as two persistent kernels (all inverse work items, then all
+ * direct work items) captured in one hipGraph at create time; the arrays
+ * must stay allocated while the ensemble exists. */
+typedef struct {
+  int ntru, nlev, nmem;
+  const void *sp_in;
+  void *gp;
+  void *sp_out;
+} exo_sht_ens_group;
+typedef struct exo_sht_ens_s *exo_sht_ens_t;
+
+int exo_sht_ens_create(exo_sht_ens_t *ens, int mode, int ngroup, const exo_sht_ens_group *groups);
+int exo_sht_ens_destroy(exo_sht_ens_t ens);
+/* one launch of the captured graph on stream */
+int exo_sht_ens_run(exo_sht_ens_t ens, void *stream);
+/* the same kernels launched directly (no graph) */
+int exo_sht_ens_run_eager(exo_sht_ens_t ens, void *stream);
+/* the same round trips as one batched exo_sht_roundtrip per group */
+int exo_sht_ens_run_groups(exo_sht_ens_t ens, void *stream);
+/* the same round trips one member at a time (4 launches per member) */
+int exo_sht_ens_run_members(exo_sht_ens_t ens, void *stream);
+/* tile variants per group, vars[2*k] inverse and vars[2*k+1] direct of
+ * group k (0 or 1, as exo_sht_set_tiles); set_tiles captures the graph
+ * again.  autotune times every combination per truncation, keeps the
+ * fastest and returns it in vars (may be NULL). */
+int exo_sht_ens_get_tiles(exo_sht_ens_t ens, int *vars);
+int exo_sht_ens_set_tiles(exo_sht_ens_t ens, const int *vars);
+int exo_sht_ens_autotune(exo_sht_ens_t ens, void *stream, int *vars);
 
 #ifdef __cplusplus
 }
