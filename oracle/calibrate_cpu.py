@@ -1,0 +1,63 @@
+"""Times the oracle's FP64 round trip against the reference's own compiled
+legmod/fftmod (oracle/_ref/libref_t21.so, T21 L10) on one host core, on the
+same seeded member, after checking that both give the same arrays.
+TEST INFRASTRUCTURE: the ratio calibrates the CPU baseline that bench.py
+measures with the oracle on the GPU box's host cores.
+
+    make -C oracle ref && python oracle/calibrate_cpu.py
+"""
+from __future__ import annotations
+
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.dirname(HERE))
+
+
+def main(reps: int = 7, calls: int = 300):
+    import oracle as O
+    from exoplasim_amd.synth import ensemble
+
+    ora, ref = O.Oracle(), O.RefT21()
+    ntru, nlev = 21, 10
+    nfl = 3 * nlev + 1
+    ngp = nfl + 1 + 2 * nlev
+    sp = np.ascontiguousarray(ensemble(ntru, nlev, 1, cfg=1)[0])
+    tables = ora.legini(ntru, ref.nlat)["_all"]
+    gp = np.zeros((ngp, ref.nlat * ref.nlon))
+    so = np.zeros((nfl, ref.nesp))
+    g_o, s_o = ora.roundtrip(ntru, nlev, sp, tables=tables)
+    ref.lib.ref_t21_roundtrip(O._p(sp), O._p(gp), O._p(so))
+    same = np.array_equal(gp.ravel(), g_o.ravel()) and np.array_equal(so.ravel(), s_o.ravel())
+
+    def per_call(fn):
+        fn()
+        t = time.perf_counter()
+        for _ in range(calls):
+            fn()
+        return (time.perf_counter() - t) / calls
+
+    ratios, t_o, t_r = [], [], []
+    for _ in range(reps):
+        a = per_call(lambda: ora.roundtrip(ntru, nlev, sp, tables=tables))
+        b = per_call(lambda: ref.lib.ref_t21_roundtrip(O._p(sp), O._p(gp), O._p(so)))
+        t_o.append(a)
+        t_r.append(b)
+        ratios.append(a / b)
+    print(f"outputs bitwise equal: {same}")
+    print(f"oracle    T21 L10 round trip: median {statistics.median(t_o) * 1e3:.3f} ms "
+          f"({1 / statistics.median(t_o):.1f} /s, 1 core)")
+    print(f"reference T21 L10 round trip: median {statistics.median(t_r) * 1e3:.3f} ms "
+          f"({1 / statistics.median(t_r):.1f} /s, 1 core)")
+    print(f"oracle / reference time: median {statistics.median(ratios):.3f} "
+          f"(min {min(ratios):.3f}, max {max(ratios):.3f}) over {reps} x {calls} calls")
+
+
+if __name__ == "__main__":
+    main()

@@ -49,3 +49,28 @@ subroutine ref_t21_plavor(p) bind(C, name="ref_t21_plavor")
   real(8) :: p
   p = plavor
 end subroutine
+
+! one member round trip with the reference's own subroutines, in the order
+! of the oracle's ora_roundtrip: sp2fc of the nfl field-levels, sp2fcdmu of
+! ln ps, dv2uv of the NLEV levels, fc2gp of everything, then gp2fc + fc2sp
+! of the nfl prognostic fields (used to time the oracle against the
+! reference on the same CPU)
+subroutine ref_t21_roundtrip(pspin, pgp, pspout) bind(C, name="ref_t21_roundtrip")
+  use pumamod
+  integer, parameter :: nfl = 3 * NLEV + 1, ngp = nfl + 1 + 2 * NLEV
+  real(8) :: pspin(NESP, nfl), pgp(NHOR, ngp), pspout(NESP, nfl)
+  real(8) :: zz(NESP, NLEV), zw(NHOR)
+  integer :: jf
+  do jf = 1, nfl
+    call sp2fc(pspin(1, jf), pgp(1, jf))
+  enddo
+  call sp2fcdmu(pspin(1, nfl), pgp(1, nfl + 1))
+  zz(:, :) = pspin(:, 1:NLEV)
+  call dv2uv(pspin(1, NLEV + 1), zz, pgp(1, nfl + 2), pgp(1, nfl + 2 + NLEV))
+  call fc2gp(pgp, NLON, NLAT * ngp)
+  do jf = 1, nfl
+    zw(:) = pgp(:, jf)
+    call gp2fc(zw, NLON, NLAT)
+    call fc2sp(zw, pspout(1, jf))
+  enddo
+end subroutine

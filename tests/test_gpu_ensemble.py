@@ -2,10 +2,12 @@
 persistent kernels) against the FP64 oracle, the per-group batched round trip
 and the one-member-at-a-time loop.
 
-Each work item of the persistent kernels is the same code as one block of the
-per-truncation kernels, so with the same tile variants all three ways of
-running the ensemble give bitwise identical results; the graph is replayed
-several times with eager work in between, and each replay is checked.
+Each work item of the persistent kernels is the same source as one block of
+the per-truncation kernels; compiled into different kernels the FMA
+contraction may differ, so the schedules agree to 1e-6 (and each with the
+oracle), while replays of one kernel are bitwise identical: the graph is
+replayed several times with eager work in between, and each replay is
+checked.
 """
 import numpy as np
 import pytest
@@ -67,11 +69,21 @@ def test_ensemble_vs_oracle_and_loops(oracle, gpu, mode):
                 eg = rel_l2(ref[k][0][m], g_o)
                 es = rel_l2(ref[k][1][m], s_o)
                 assert eg < 1e-6 and es < 1e-6, (ntru, m, eg, es)
-        # graph replays with eager work in between, then the other schedules
-        for run in (ens.run, ens.run, ens.run_eager, ens.run_groups, ens.run_members, ens.run):
+        # graph replays with eager work in between: bitwise
+        for run in (ens.run, ens.run, ens.run_eager, ens.run):
             _poison(ens)
             run()
             assert _same(_outputs(ens), ref), run.__name__
+        # the per-truncation kernels: batched and one member at a time agree
+        # bitwise with each other and to 1e-6 with the persistent kernels
+        _poison(ens)
+        ens.run_groups()
+        grp = _outputs(ens)
+        _poison(ens)
+        ens.run_members()
+        assert _same(_outputs(ens), grp)
+        for k in range(len(GROUPS)):
+            assert rel_l2(grp[k][0], ref[k][0]) < 1e-6 and rel_l2(grp[k][1], ref[k][1]) < 1e-6
     finally:
         ens.close()
 
@@ -101,7 +113,7 @@ def test_ensemble_tiles_and_autotune(oracle, gpu):
             g_o, s_o = oracle.roundtrip(ntru, nlev, host[k][0])
             assert rel_l2(tuned[k][0][0], g_o) < 1e-6 and rel_l2(tuned[k][1][0], s_o) < 1e-6
         _poison(ens)
-        ens.run_members()
+        ens.run()
         assert _same(_outputs(ens), tuned)
     finally:
         ens.close()
