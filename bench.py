@@ -41,6 +41,9 @@ def parse():
                     help="keep the default tiles instead of timing the variants")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the T42 L20 headline (skip ensemble, T127, emulator)")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (the "
+                         "reference kernel that scales the profiler's byte counters)")
     ap.add_argument("--emulator-columns", type=int, default=2_000_000)
     ap.add_argument("--emulator-epochs", type=int, default=6)
     return ap.parse_args()
@@ -279,6 +282,12 @@ def main():
     from exoplasim_amd import sht
     from exoplasim_amd.synth import ensemble
 
+    if args.copy_probe_mb:
+        src = torch.ones(args.copy_probe_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     nmem = args.members
     plan = sht.Plan(NTRU)
     nfl = 3 * NLEV + 1

@@ -240,6 +240,112 @@ __global__ void k_cast_rows(const float* src, int M, int nin, int K, uint16_t* d
 
 inline unsigned nb(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// ---------------------------------------------------------------------------
+// fused inference: features -> 4 layers -> heating rates and OLR, one block
+// per 32 columns, activations in LDS, weights streamed from L2
+// ---------------------------------------------------------------------------
+constexpr int EM_NLEV = 20, EM_KIN = 64, EM_NH = 256, EM_NOUT = 21, EM_NOUTP = 32;
+constexpr int EM_MT = 32;             // columns per block
+constexpr int EM_LDH = EM_NH + 8;     // padded LDS row (bf16) of a hidden layer
+constexpr int EM_LDX = EM_KIN + 8;
+
+// one hidden layer: Hout[32][256] = gelu(Hin[32][K] W^T + b); wave w owns
+// output columns [64w, 64w + 64)
+template <int K, int LDI>
+__device__ __forceinline__ void em_hidden(const uint16_t* hin, const uint16_t* W, const float* b,
+                                          uint16_t* hout) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int q = lane >> 4, c = lane & 15;
+  f32x4 acc[2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const uint16_t* wr = W + (long)(wv * 64 + c) * K + q * 8;
+#pragma unroll 2
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    V8 a[2], bb[4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) a[i].q = *(const uint4*)(hin + (i * 16 + c) * LDI + k0 + q * 8);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bb[j].q = *(const uint4*)(wr + (long)j * 16 * K + k0);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], bb[j], acc[i][j]);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int n = wv * 64 + j * 16 + c;
+    const float bn = b[n];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) hout[(i * 16 + 4 * q + r) * EM_LDH + n] = f2bf(gelu(acc[i][j][r] + bn));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_emulate(exo_lw_net net, int ncol, const float* dt,
+                                                 const float* dq, const float* dp,
+                                                 const float* co2, float* heat, float* olr) {
+  __shared__ __align__(16) uint16_t sx[EM_MT * EM_LDX];
+  __shared__ __align__(16) uint16_t sh0[EM_MT * EM_LDH];
+  __shared__ __align__(16) uint16_t sh1[EM_MT * EM_LDH];
+  const int tid = threadIdx.x;
+  const int col0 = blockIdx.x * EM_MT;
+  // features (exoplasim_amd.radiation.features), column index fastest
+  for (int idx = tid; idx < EM_MT * EM_KIN; idx += 256) {
+    const int m = idx % EM_MT, f = idx / EM_MT;
+    const int col = min(col0 + m, ncol - 1);
+    float v = 0.f;
+    if (f < EM_NLEV)
+      v = (dt[(long)f * ncol + col] - 250.0f) / 50.0f;
+    else if (f < 2 * EM_NLEV)
+      v = (logf(fmaxf(dq[(long)(f - EM_NLEV) * ncol + col], 1e-12f)) + 10.0f) / 3.5f;
+    else if (f == 2 * EM_NLEV)
+      v = (dt[(long)EM_NLEV * ncol + col] - 265.0f) / 40.0f;
+    else if (f == 2 * EM_NLEV + 1)
+      v = logf(dp[col] / 1e5f) / 1.3f;
+    else if (f == 2 * EM_NLEV + 2)
+      v = (logf(co2[col]) - 5.75f) / 3.3f;
+    sx[m * EM_LDX + f] = f2bf(v);
+  }
+  __syncthreads();
+  em_hidden<EM_KIN, EM_LDX>(sx, (const uint16_t*)net.w[0], net.b[0], sh0);
+  __syncthreads();
+  em_hidden<EM_NH, EM_LDH>(sh0, (const uint16_t*)net.w[1], net.b[1], sh1);
+  __syncthreads();
+  em_hidden<EM_NH, EM_LDH>(sh1, (const uint16_t*)net.w[2], net.b[2], sh0);
+  __syncthreads();
+  // output layer 32 x 32: wave w takes row tile w>>1, column tile w&1
+  const int lane = tid & 63, wv = tid >> 6;
+  const int q = lane >> 4, c = lane & 15;
+  const int rt = wv >> 1, ct = wv & 1;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  const uint16_t* W4 = (const uint16_t*)net.w[3] + (long)(ct * 16 + c) * EM_NH + q * 8;
+#pragma unroll 4
+  for (int k0 = 0; k0 < EM_NH; k0 += 32) {
+    V8 a, bb;
+    a.q = *(const uint4*)(sh0 + (rt * 16 + c) * EM_LDH + k0 + q * 8);
+    bb.q = *(const uint4*)(W4 + k0);
+    acc = mfma(a, bb, acc);
+  }
+  const int n = ct * 16 + c;
+  if (n < EM_NOUT) {
+    const float bn = net.b[3][n], sd = net.ystd[n], mu = net.ymean[n];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int col = col0 + rt * 16 + 4 * q + r;
+      if (col >= ncol) continue;
+      const float y = (acc[r] + bn) * sd + mu;
+      if (n < EM_NLEV)
+        heat[(long)n * ncol + col] = y / (dp[col] / 1e5f);
+      else
+        olr[col] = y;
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" int exo_mlp_forward(const void* X, const void* W, const float* b, int M, int K, int N,
@@ -288,6 +394,20 @@ extern "C" int exo_mlp_cast_weights(const float* w, int nout, int nin, int N, in
   if (!w || !Wb || !Wtb || nout > N || nin > K) return EXO_RAD_ERR_ARG;
   k_cast_w<<<nb((long)N * K, 256), 256, 0, (hipStream_t)stream>>>(w, nout, nin, N, K,
                                                                   (uint16_t*)Wb, (uint16_t*)Wtb);
+  return hipGetLastError() == hipSuccess ? EXO_RAD_OK : EXO_RAD_ERR_HIP;
+}
+
+extern "C" int exo_lw_emulate(const exo_lw_net* net, int ncol, const float* dt, const float* dq,
+                              const float* dp, const float* dqco2, float* heat, float* olr,
+                              void* stream) {
+  if (!net || ncol < 0 || net->nlev != EM_NLEV) return EXO_RAD_ERR_ARG;
+  if (ncol == 0) return EXO_RAD_OK;
+  if (!dt || !dq || !dp || !dqco2 || !heat || !olr || !net->ymean || !net->ystd)
+    return EXO_RAD_ERR_ARG;
+  for (int l = 0; l < 4; ++l)
+    if (!net->w[l] || !net->b[l]) return EXO_RAD_ERR_ARG;
+  k_emulate<<<nb(ncol, EM_MT), 256, 0, (hipStream_t)stream>>>(*net, ncol, dt, dq, dp, dqco2,
+                                                               heat, olr);
   return hipGetLastError() == hipSuccess ? EXO_RAD_OK : EXO_RAD_ERR_HIP;
 }
 

@@ -19,37 +19,9 @@
 //                 FP32 accumulate
 // FFTs run in FP64 for MODE_F64 and FP32 otherwise.
 #include "sht_bodies.h"
+#include "sht_launch.h"
 
 namespace exo {
-
-template <int T, int MODE, int KIND, bool FUSED, int V>
-__global__ __launch_bounds__(256) void k_inv(InvArgs a, const void* tabA0, const void* tabA1,
-                                             long plane, const void* tabv, const void* twp) {
-  __shared__ __align__(16) char lds[InvLds<T, MODE, FUSED, V>::BYTES];
-  inv_body<T, MODE, KIND, FUSED, V>(a, blockIdx.x, tabA0, tabA1, plane, tabv, twp, lds);
-}
-
-template <int T, int MODE, bool FUSED, int V>
-__global__ __launch_bounds__(256) void k_fwd(FwdArgs a, const void* tabv, const void* twp,
-                                             void* scratch) {
-  __shared__ __align__(16) char lds[FwdLds<T, MODE, FUSED, V>::BYTES];
-  fwd_body<T, MODE, FUSED, V>(a, blockIdx.x, tabv, twp, scratch, lds);
-}
-
-// sum of the per-pair-block partial direct transforms in a fixed order
-template <typename R>
-__global__ void k_reduce_partials(const R* scratch, R* sp, int npb, int nmem, int nfield,
-                                  int nrsp, long sp_stride, int dst0) {
-  const long total = (long)nmem * nfield * nrsp;
-  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  const int x = (int)(i % nrsp);
-  const int f = (int)((i / nrsp) % nfield);
-  const int mem = (int)(i / ((long)nrsp * nfield));
-  R s = 0;
-  for (int b = 0; b < npb; ++b) s += scratch[(long)b * total + i];
-  sp[mem * sp_stride + (long)(dst0 + f) * nrsp + x] = s;
-}
 
 // ---------------------------------------------------------------------------
 // FFT of independent rows (in place), two real rows per complex FFT
@@ -117,113 +89,6 @@ __global__ __launch_bounds__(256) void k_fft_rows(R* a, long lot, const C2<R>* t
         x1[2 * k + 1] = (zn.x - zk.x) * sc;
       }
     }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// launchers
-// ---------------------------------------------------------------------------
-template <int T, int MODE, int V>
-static int inv_tv(const Tables& tb, bool fused, const InvArgs& a, hipStream_t s) {
-  const long nblk = inv_items<T, MODE, V>(a);
-  if (nblk <= 0) return 0;
-  const void *A0 = nullptr, *A1 = nullptr;
-  const int t0 = (a.kind == INV_UV) ? TQV : a.table;
-  if (MODE == MODE_BF16X3) {
-    A0 = tb.inv_bf[t0];
-    A1 = tb.inv_bf[TQU];
-  } else if (MODE == MODE_F32MFMA) {
-    A0 = tb.inv_f32[t0];
-    A1 = tb.inv_f32[TQU];
-  }
-  const void* tv = (MODE == MODE_F64) ? (const void*)tb.filt64 : (const void*)tb.filt32;
-  const void* tw = (MODE == MODE_F64) ? (const void*)tb.tw64 : (const void*)tb.tw32;
-  dim3 grid((unsigned)nblk), block(256);
-  if (a.kind == INV_UV) {
-    if (fused)
-      k_inv<T, MODE, INV_UV, true, V><<<grid, block, 0, s>>>(a, A0, A1, tb.inv_elems, tv, tw);
-    else
-      k_inv<T, MODE, INV_UV, false, V><<<grid, block, 0, s>>>(a, A0, A1, tb.inv_elems, tv, tw);
-  } else {
-    if (fused)
-      k_inv<T, MODE, INV_SCALAR, true, V><<<grid, block, 0, s>>>(a, A0, A1, tb.inv_elems, tv, tw);
-    else
-      k_inv<T, MODE, INV_SCALAR, false, V><<<grid, block, 0, s>>>(a, A0, A1, tb.inv_elems, tv, tw);
-  }
-  return hipGetLastError() == hipSuccess ? 0 : -3;
-}
-
-// the unfused (drop-in) transforms always use the default tiles
-template <int T, int MODE>
-static int inv_t(const Tables& tb, bool fused, const InvArgs& a, hipStream_t s) {
-  if (fused && a.var == 1) return inv_tv<T, MODE, 1>(tb, fused, a, s);
-  return inv_tv<T, MODE, 0>(tb, fused, a, s);
-}
-
-template <int T, int MODE>
-static size_t fwd_scratch_t(int nmem, int nfield) {
-  using C = FwdCfg<T, MODE>;
-  if (C::NPB == 1) return 0;
-  return (size_t)C::NPB * nmem * nfield * TS<T>::NRSP * sizeof(RealT<MODE>);
-}
-
-template <int T, int MODE, int V>
-static int fwd_tv(const Tables& tb, bool fused, const FwdArgs& a, void* scratch, size_t sbytes,
-                  hipStream_t s) {
-  using C = FwdCfg<T, MODE, V>;
-  using R = RealT<MODE>;
-  const long nblk = fwd_items<T, MODE, V>(a);
-  if (nblk <= 0) return 0;
-  if (C::NPB > 1 && (scratch == nullptr || sbytes < fwd_scratch_t<T, MODE>(a.nmem, a.nfield)))
-    return -4;
-  const void* tv = (MODE == MODE_F64) ? (const void*)tb.filt64 : (const void*)tb.filt32;
-  const void* tw = (MODE == MODE_F64) ? (const void*)tb.tw64 : (const void*)tb.tw32;
-  dim3 grid((unsigned)nblk), block(256);
-  if (fused)
-    k_fwd<T, MODE, true, V><<<grid, block, 0, s>>>(a, tv, tw, scratch);
-  else
-    k_fwd<T, MODE, false, V><<<grid, block, 0, s>>>(a, tv, tw, scratch);
-  if (C::NPB > 1) {
-    const long total = (long)a.nmem * a.nfield * TS<T>::NRSP;
-    k_reduce_partials<R><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
-        (const R*)scratch, (R*)a.sp, C::NPB, a.nmem, a.nfield, TS<T>::NRSP, a.sp_stride, a.dst0);
-  }
-  return hipGetLastError() == hipSuccess ? 0 : -3;
-}
-
-template <int T, int MODE>
-static int fwd_t(const Tables& tb, bool fused, const FwdArgs& a, void* scratch, size_t sbytes,
-                 hipStream_t s) {
-  if (fused && a.var == 1) return fwd_tv<T, MODE, 1>(tb, fused, a, scratch, sbytes, s);
-  return fwd_tv<T, MODE, 0>(tb, fused, a, scratch, sbytes, s);
-}
-
-#define EXO_DISPATCH_T(fn, ...)                                   \
-  switch (tb.ntru) {                                              \
-    case 21: return fn<21, MODE_X>(__VA_ARGS__);                  \
-    case 42: return fn<42, MODE_X>(__VA_ARGS__);                  \
-    case 63: return fn<63, MODE_X>(__VA_ARGS__);                  \
-    case 127: return fn<127, MODE_X>(__VA_ARGS__);                \
-    default: return -2;                                           \
-  }
-
-template <int MODE_X>
-static int inv_mode(const Tables& tb, bool fused, const InvArgs& a, hipStream_t s) {
-  EXO_DISPATCH_T(inv_t, tb, fused, a, s)
-}
-template <int MODE_X>
-static int fwd_mode(const Tables& tb, bool fused, const FwdArgs& a, void* scr, size_t sb,
-                    hipStream_t s) {
-  EXO_DISPATCH_T(fwd_t, tb, fused, a, scr, sb, s)
-}
-template <int MODE_X>
-static size_t scr_mode(const Tables& tb, int nmem, int nfield) {
-  switch (tb.ntru) {
-    case 21: return fwd_scratch_t<21, MODE_X>(nmem, nfield);
-    case 42: return fwd_scratch_t<42, MODE_X>(nmem, nfield);
-    case 63: return fwd_scratch_t<63, MODE_X>(nmem, nfield);
-    case 127: return fwd_scratch_t<127, MODE_X>(nmem, nfield);
-    default: return 0;
   }
 }
 

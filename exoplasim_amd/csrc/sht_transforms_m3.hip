@@ -1,0 +1,8 @@
+// sht_transforms_m3.hip -- the transform kernels of MODE_BF16X3.
+#include "sht_transforms_impl.h"
+
+namespace exo {
+template int inv_mode<MODE_BF16X3>(const Tables&, bool, const InvArgs&, hipStream_t);
+template int fwd_mode<MODE_BF16X3>(const Tables&, bool, const FwdArgs&, void*, size_t, hipStream_t);
+template size_t scr_mode<MODE_BF16X3>(const Tables&, int, int);
+}  // namespace exo

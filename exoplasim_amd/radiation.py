@@ -48,6 +48,11 @@ class _Out(ctypes.Structure):
     _fields_ = [(k, c_void_p) for k in ("dftu", "dftd", "heat", "olr")]
 
 
+class _Net(ctypes.Structure):
+    _fields_ = [("nlev", c_int), ("w", c_void_p * 4), ("b", c_void_p * 4), ("ymean", c_void_p),
+                ("ystd", c_void_p)]
+
+
 def _proto():
     L = lib()
     if not getattr(L, "_rad_proto", False):
@@ -66,6 +71,8 @@ def _proto():
         L.exo_mlp_cast_weights.argtypes = [fp, c_int, c_int, c_int, c_int, vp, vp, vp]
         L.exo_mlp_cast_rows.restype = c_int
         L.exo_mlp_cast_rows.argtypes = [fp, c_int, c_int, c_int, vp, vp]
+        L.exo_lw_emulate.restype = c_int
+        L.exo_lw_emulate.argtypes = [POINTER(_Net), c_int, fp, fp, fp, fp, fp, fp, vp]
         L._rad_proto = True
     return L
 
@@ -289,6 +296,7 @@ class LwEmulator:
                                                                    requires_grad=True)]
         self.ymean = torch.zeros(NOUT, device="cuda")
         self.ystd = torch.ones(NOUT, device="cuda")
+        self._net = None  # bf16 inference weights, rebuilt after training
 
     def targets(self, heat, olr, cols):
         """normalised targets: heating * ps/1e5 (removes the 1/ps of the
@@ -301,8 +309,43 @@ class LwEmulator:
     def raw(self, x):
         return emulator_op().apply(x, *self.params)
 
+    def _freeze(self):
+        """bf16 padded copies of the weights for the fused inference kernel"""
+        import torch
+
+        L = _proto()
+        s = _stream()
+        ws = []
+        for (nin, nout), (K, N), w in zip(LAYERS, PADDED, self.params[0::2]):
+            W = torch.empty((N, K), dtype=torch.bfloat16, device="cuda")
+            Wt = torch.empty((K, N), dtype=torch.bfloat16, device="cuda")
+            _chk(L.exo_mlp_cast_weights(_ptr(w.detach().contiguous()), nout, nin, N, K, _ptr(W),
+                                        _ptr(Wt), s), "cast_weights")
+            ws.append(W)
+        bs = [b.detach().float().contiguous() for b in self.params[1::2]]
+        ym, ys = self.ymean.float().contiguous(), self.ystd.float().contiguous()
+        net = _Net(self.nlev, (c_void_p * 4)(*[_ptr(w) for w in ws]),
+                   (c_void_p * 4)(*[_ptr(b) for b in bs]), _ptr(ym), _ptr(ys))
+        self._net = (net, ws, bs, ym, ys)  # keeps the buffers alive
+
     def predict(self, cols):
-        """heating (nlev, ncol) K/day and OLR (ncol) of device columns."""
+        """heating (nlev, ncol) K/day and OLR (ncol) of device columns, in
+        one kernel (exo_lw_emulate)."""
+        import torch
+
+        if self._net is None:
+            self._freeze()
+        c = {k: cols[k].float().contiguous() for k in ("dt", "dq", "dp", "dqco2")}
+        ncol = c["dp"].shape[0]
+        heat = torch.empty((self.nlev, ncol), dtype=torch.float32, device=c["dp"].device)
+        olr = torch.empty(ncol, dtype=torch.float32, device=c["dp"].device)
+        _chk(_proto().exo_lw_emulate(ctypes.byref(self._net[0]), ncol, _ptr(c["dt"]),
+                                     _ptr(c["dq"]), _ptr(c["dp"]), _ptr(c["dqco2"]), _ptr(heat),
+                                     _ptr(olr), _stream()), "exo_lw_emulate")
+        return heat, olr
+
+    def predict_layers(self, cols):
+        """the same through the per-layer training kernels (features in torch)"""
         import torch
 
         with torch.no_grad():
@@ -315,6 +358,7 @@ class LwEmulator:
         """Adam on the normalised targets; x (n, 43), y (n, 21) device tensors."""
         import torch
 
+        self._net = None
         self.ymean = y.mean(dim=0)
         self.ystd = y.std(dim=0).clamp_min(1e-6)
         yn = (y - self.ymean) / self.ystd

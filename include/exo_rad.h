@@ -68,6 +68,26 @@ int exo_mlp_cast_weights(const float *w, int nout, int nin, int N, int K, void *
 /* fp32 rows src[M][nin] -> zero-padded bf16 dst[M][K] */
 int exo_mlp_cast_rows(const float *src, int M, int nin, int K, void *dst, void *stream);
 
+/* A trained emulator for inference: bf16 weights W1[256][64], W2[256][256],
+ * W3[256][256], W4[32][256] (zero padded, as exo_mlp_cast_weights writes
+ * them), fp32 biases, and the output normalisation ymean/ystd (21 values).
+ * nlev must be 20. */
+typedef struct {
+  int nlev;
+  const void *w[4];
+  const float *b[4];
+  const float *ymean, *ystd;
+} exo_lw_net;
+/* The learned long-wave column in one kernel, in place of lwr
+ * (src/radmod.f90:2122) and the heating rate (src/radmod.f90:1107-1108):
+ * from fp32 device columns in the model layout field(NHOR, lev) -- dt
+ * (nlev+1 rows, the last the surface), dq (nlev rows), dp (1 row), dqco2 (its
+ * first row) -- to heat (nlev rows, K/day) and olr (1 row, W/m2).  Features,
+ * the four layers (bf16 MFMA, fp32 accumulate) and the output scaling run
+ * per block of 32 columns with the activations in LDS. */
+int exo_lw_emulate(const exo_lw_net *net, int ncol, const float *dt, const float *dq,
+                   const float *dp, const float *dqco2, float *heat, float *olr, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,6 +75,43 @@ def test_emulator_gradients_vs_oracle(gpu):
         assert e < 5e-3, (i, e)
 
 
+def test_emulator_fused_inference(gpu):
+    """the one-kernel inference (exo_lw_emulate) against the per-layer
+    kernels and the FP64 oracle network, on a column count that leaves a
+    partial block"""
+    import torch
+
+    import mlp_oracle as MO
+    from exoplasim_amd.columns import make_columns
+    from exoplasim_amd.radiation import LwEmulator, columns_to_device, features
+
+    print("emulator fused inference", flush=True)
+    em = LwEmulator(seed=5)
+    g = torch.Generator(device="cpu").manual_seed(9)
+    em.ymean = (torch.randn(21, generator=g) * 3.0).cuda()
+    em.ystd = (torch.rand(21, generator=g) + 0.5).cuda()
+    with torch.no_grad():
+        for b in em.params[1::2]:
+            b.copy_(torch.randn(b.shape, generator=g) * 0.1)
+    cols = columns_to_device(make_columns(1000, 20, 21), torch.float32)
+    h1, o1 = em.predict(cols)
+    h2, o2 = em.predict_layers(cols)
+    torch.cuda.synchronize()
+    rel = lambda a, b: float(torch.linalg.norm(a - b) / torch.linalg.norm(b))  # noqa: E731
+    assert torch.isfinite(h1).all() and torch.isfinite(o1).all()
+    assert rel(h1, h2) < 2e-3 and rel(o1, o2) < 2e-3, (rel(h1, h2), rel(o1, o2))
+    # oracle: same features, FP64 network with the kernels' bf16 rounding
+    params = [p.detach().double().cpu().numpy() for p in em.params]
+    y, _ = MO.forward(features(cols, 20).double().cpu().numpy(), params)
+    y = y * em.ystd.double().cpu().numpy() + em.ymean.double().cpu().numpy()
+    ps = cols["dp"].double().cpu().numpy() / 1e5
+    h_o = (y[:, :20] / ps[:, None]).T
+    e_h = np.linalg.norm(h1.double().cpu().numpy() - h_o) / np.linalg.norm(h_o)
+    e_o = np.linalg.norm(o1.double().cpu().numpy() - y[:, 20]) / np.linalg.norm(y[:, 20])
+    print(f"fused vs oracle: heating {e_h:.2e}, olr {e_o:.2e}", flush=True)
+    assert e_h < 2e-3 and e_o < 2e-3
+
+
 def test_emulator_training_and_gain(gpu):
     """train on columns produced by the GPU reference column, then compare
     heating-rate RMS and throughput with the column it replaces"""
@@ -119,6 +156,8 @@ def test_emulator_training_and_gain(gpu):
 
     t_col = timeit(lambda: col(test))
     t_em = timeit(lambda: em.predict(test))
+    t_em_layers = timeit(lambda: em.predict_layers(test))
+    print(f"emulator per-layer kernels {8192 / t_em_layers:.3e} col/s", flush=True)
     print(f"plain column {8192 / t_col:.3e} col/s, emulator {8192 / t_em:.3e} col/s, "
           f"gain x{t_col / t_em:.2f}", flush=True)
     # the network learns the column: far better than predicting the mean
