@@ -136,18 +136,24 @@ __global__ __launch_bounds__(128) void k_lwr(LwrArgs<R> a) {
       ztau[j2] = tcs * ztaucc;
     }
     if (jlev == 1) {
-      R ztau0 = R(1.) - zero;
+      // Effective radiation of each layer.  Near ztau0 = 1 - 1e-6 both
+      // 1/(1 - ztau0) and 1/ln(ztau0) are ~1e6 and cancel; the function is
+      // well conditioned, its evaluation is not, so this block runs in FP64
+      // for either column precision (NLEV evaluations per column).
+      const double zd = 1.e-6;
+      double ztau0 = 1. - zd;
       for (int j2 = 1; j2 <= NLEV; ++j2) {
         const int jp2 = j2 + 1;
-        ztau0 = ztaucs[j2] / ztau0 * ztaucc0[j2] * R(a.tpofmt);
-        ztau0 = fmin(R(1.) - zero, fmax(zero, ztau0));
+        ztau0 = (double)ztaucs[j2] / ztau0 * (double)ztaucc0[j2] * a.tpofmt;
+        ztau0 = fmin(1. - zd, fmax(zd, ztau0));
         if ((zst4[j2] - zst4h[j2]) * (zst4[j2] - zst4h[jp2]) > 0) {
           zbd[j2] = R(0.5) * zst4[j2] + R(0.25) * (zst4h[j2] + zst4h[jp2]);
           zbu[j2] = zbd[j2];
         } else {
-          zbd[j2] = (zst4h[jp2] - ztau0 * zst4h[j2]) / (R(1.) - ztau0) -
-                    (zst4h[j2] - zst4h[jp2]) / ln(ztau0);
-          zbu[j2] = zst4h[j2] + zst4h[jp2] - zbd[j2];
+          const double h1 = zst4h[j2], h2 = zst4h[jp2];
+          const double bd = (h2 - ztau0 * h1) / (1. - ztau0) - (h1 - h2) / log(ztau0);
+          zbd[j2] = (R)bd;
+          zbu[j2] = (R)(h1 + h2 - bd);
         }
         ztau0 = ztaucs[j2];
       }
