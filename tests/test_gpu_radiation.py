@@ -34,9 +34,14 @@ def test_lwr_column_vs_oracle(oracle, gpu):
     heat_kday = heat * 86400.0
     dcols = columns_to_device(cols)
     # fp64: 1e-10, the reference's own -O0/-O2 spread where the effective
-    # radiation cancels near ztau0 = 1 - 1e-6 (see test_oracle.LWR_TOL);
-    # fp32 interface and arithmetic: 2e-4
-    for fp64, tol in ((True, 1e-10), (False, 2e-4)):
+    # radiation cancels near ztau0 = 1 - 1e-6 (see test_oracle.LWR_TOL).
+    # fp32: 5e-3.  In isothermal layers (profiles clipped at 150 K) the sign
+    # test that picks the effective-radiation formula (src/radmod.f90:2380)
+    # is decided by rounding, and the two formulas differ; an FP32 C
+    # restatement of the column differs from the FP64 oracle by 2.5e-3 on
+    # these columns, and the reference's own single-precision build
+    # (oracle/_ref/libref_rad_t21_r4.so) from its FP64 build by 4.7e-2.
+    for fp64, tol in ((True, 1e-10), (False, 5e-3)):
         col = LwrColumn(nlev, fp64=fp64)
         h, o, u, d = col(dcols, fluxes=True)
         torch.cuda.synchronize()
@@ -47,7 +52,7 @@ def test_lwr_column_vs_oracle(oracle, gpu):
         e_h = np.linalg.norm(h - heat_kday) / np.linalg.norm(heat_kday)
         e_o = np.abs(o - olr).max() / np.abs(olr).max()
         print(f"fp64={fp64}: flux {e_u:.2e} heating {e_h:.2e} olr {e_o:.2e}", flush=True)
-        assert e_u < tol and e_h < 10 * tol and e_o < tol
+        assert e_u < tol and e_h < (10 * tol if fp64 else tol) and e_o < tol
 
 
 def test_emulator_gradients_vs_oracle(gpu):
