@@ -236,7 +236,8 @@ def bench_emulator(args, ora):
     sig = float(torch.sqrt(torch.mean(h_ref ** 2)))
     rms_olr = float(torch.sqrt(torch.mean((o_em - o_ref) ** 2)))
     t_col = time_loop(lambda: col(test), 20, 3) / 20
-    t_em = time_loop(lambda: em.predict(test), 20, 3) / 20
+    outb = (torch.empty_like(h_em), torch.empty_like(o_em))
+    t_em = time_loop(lambda: em.predict(test, out=outb), 200, 10) / 200
     # CPU: the oracle column (C restatement of lwr) on host threads
     import oracle as O
 

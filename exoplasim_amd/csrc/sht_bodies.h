@@ -56,7 +56,8 @@ struct InvLds {
   using R = RealT<MODE>;
   using C = InvCfg<T, MODE, V>;
   static constexpr size_t FCL = sizeof(NS2<R>) * C::LP * C::FB * TS<T>::NM;
-  static constexpr size_t FBUF = FUSED ? sizeof(C2<R>) * FftBatch<TS<T>::NLON>::NF * TS<T>::NLON : 0;
+  static constexpr size_t FBUF =
+      FUSED ? sizeof(C2<R>) * fft_buf_elems(FftBatch<TS<T>::NLON>::NF * TS<T>::NLON) : 0;
   static constexpr size_t TWS = FUSED ? sizeof(C2<R>) * TS<T>::NLON : 0;
   static constexpr size_t BYTES = FCL + FBUF + TWS;
 };
@@ -143,33 +144,65 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     const int q = lane >> 4, c = lane & 15;
     // column c of B: SCALAR field g*FB + c/2; UV level g*FB/2 + c/4 of
     // vorticity ((c/2)&1 == 0) or divergence; component c&1
-    const R* bp = nullptr;
+    const R* bp = nullptr;  // F32MFMA: the column's fp32 coefficients
+    const uint16_t* pbp = nullptr;  // BF16X3: the column's pre-split operand
     bool isvor = false;
+    int pf = -1;  // field of column c
     if (KIND == INV_SCALAR) {
       const int field = g * FB + (c >> 1);
-      if ((c >> 1) < FB && field < a.nfield) bp = spm + (long)(a.src0 + field) * S::NRSP + (c & 1);
+      if ((c >> 1) < FB && field < a.nfield) pf = a.src0 + field;
     } else {
       const int lvl = g * (FB / 2) + (c >> 2);
       const int s = (c >> 1) & 1;
       if ((c >> 2) < FB / 2 && lvl < a.nlev) {
-        bp = spm + (long)((s ? a.div0 : a.vor0) + lvl) * S::NRSP + (c & 1);
+        pf = (s ? a.div0 : a.vor0) + lvl;
         isvor = (s == 0) && ((c & 1) == 0);
       }
     }
+    if (pf >= 0) {
+      if constexpr (MODE == MODE_BF16X3)
+        pbp = (const uint16_t*)a.bpk + mem * a.bpk_stride +
+              (long)(pf - a.bpk_f0) * (pk_blocks(T) * PK_BLK) + (c & 1) * 32 + q * 8;
+      else
+        bp = spm + (long)pf * S::NRSP + (c & 1);
+    }
     const R pv = (R)a.plavor;
-    // n offset of B element j (bf16: 8 consecutive n per lane; f32: n = 4j + q)
-    auto noff = [&](int j) { return MODE == MODE_BF16X3 ? q * 8 + j : j * 4 + q; };
-    auto fetch = [&](int m, int kb, float* br, uint4* a0, uint4* a1) {
-      const int w0 = w0_of(T, m);
+    // B fragment of one item: 8 fp32 values (F32MFMA, n = 4j + q) or the
+    // three bf16 planes of 8 consecutive n (BF16X3, n = 8q + j)
+    struct BFrag {
+      float f[8];
+      BF8 p[3];
+    };
+    auto fetch = [&](int m, int kb, BFrag& br, uint4* a0, uint4* a1) {
+      if constexpr (MODE == MODE_BF16X3) {
+        if (pbp) {
+          const uint16_t* src = pbp + (pk_blkoff(T, m) + kb) * PK_BLK;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int n = m + kb * 32 + noff(j);
-        R x = R(0);
-        if (bp && n <= T) {
-          x = bp[2 * (w0 + n - m)];
-          if (isvor && m == 0 && n == 1) x = x - pv;
+          for (int p = 0; p < 3; ++p) br.p[p].q = *(const uint4*)(src + p * 64);
+          if (isvor && m == 0 && kb == 0 && q == 0) {
+            // absolute -> relative vorticity at (m=0, n=1, re): the split is
+            // exact, so the sum of the planes is the fp32 coefficient
+            const float x = __uint_as_float((uint32_t)br.p[0].s[1] << 16) +
+                            __uint_as_float((uint32_t)br.p[1].s[1] << 16) +
+                            __uint_as_float((uint32_t)br.p[2].s[1] << 16);
+            split3(x - (float)pv, br.p[0].s[1], br.p[1].s[1], br.p[2].s[1]);
+          }
+        } else {
+#pragma unroll
+          for (int p = 0; p < 3; ++p) br.p[p].q = uint4{0u, 0u, 0u, 0u};
         }
-        br[j] = (float)x;
+      } else {
+        const int w0 = w0_of(T, m);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int n = m + kb * 32 + j * 4 + q;
+          R x = R(0);
+          if (bp && n <= T) {
+            x = bp[2 * (w0 + n - m)];
+            if (isvor && m == 0 && n == 1) x = x - pv;
+          }
+          br.f[j] = (float)x;
+        }
       }
       const long blk = ((long)(pb * NM + m) * S::KB + kb) * 512;
       if constexpr (MODE == MODE_BF16X3) {
@@ -193,29 +226,27 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
         }
       }
     };
-    auto mul = [&](const float* br, const uint4* a, f32x4 acc) -> f32x4 {
+    auto mul = [&](const BFrag& br, const uint4* a, f32x4 acc) -> f32x4 {
       if constexpr (MODE == MODE_BF16X3) {
-        BF8 bh, bm, bl, ah, am, al;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) split3(br[j], bh.s[j], bm.s[j], bl.s[j]);
+        BF8 ah, am, al;
         ah.q = a[0];
         am.q = a[1];
         al.q = a[2];
-        acc = mfma_bf16(ah, bl, acc);
-        acc = mfma_bf16(am, bm, acc);
-        acc = mfma_bf16(al, bh, acc);
-        acc = mfma_bf16(ah, bm, acc);
-        acc = mfma_bf16(am, bh, acc);
-        acc = mfma_bf16(ah, bh, acc);
+        acc = mfma_bf16(ah, br.p[2], acc);
+        acc = mfma_bf16(am, br.p[1], acc);
+        acc = mfma_bf16(al, br.p[0], acc);
+        acc = mfma_bf16(ah, br.p[1], acc);
+        acc = mfma_bf16(am, br.p[0], acc);
+        acc = mfma_bf16(ah, br.p[0], acc);
       } else {
         const float* af = (const float*)a;
 #pragma unroll
         for (int j = 0; j < 8; ++j)
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[j], br[j], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[j], br.f[j], acc, 0, 0, 0);
       }
       return acc;
     };
-    float br[8], br2[8];
+    BFrag br, br2;
     uint4 a0[3], a1[3], a02[3], a12[3];
     int m = wv, kb = 0;
     if (m <= T) fetch(m, 0, br, a0, a1);
@@ -269,8 +300,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
       }
       m = m2;
       kb = kb2;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) br[j] = br2[j];
+      br = br2;
 #pragma unroll
       for (int p = 0; p < 3; ++p) {
         a0[p] = a02[p];
@@ -352,19 +382,18 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     // Im -> south row (imag of m=0 ignored, as fc2gp does) ----
     for (int f0 = 0; f0 < LP * FB; f0 += NF) {
       const int nf = min(NF, LP * FB - f0);
+      // z_k = fc_N(k) + i fc_S(k) for k <= N/2 and its Hermitian extension
+      // above; zero outside the truncation and Im of m = 0 ignored (as
+      // fc2gp does).  Branch-free: one LDS read and selects per element.
       auto load = [&](int f, int k) -> C2<R> {
-        const int gi = f0 + f, pl = gi / FB, fl = gi % FB;
-        const NS2<R>* e = fcl + (pl * FB + fl) * NM;
-        if (k == 0) return C2<R>{e[0].n.x, e[0].s.x};
-        if (k <= T) {
-          const NS2<R> v = e[k];
-          return C2<R>{v.n.x - v.s.y, v.n.y + v.s.x};
-        }
-        if (k >= N - T) {
-          const NS2<R> v = e[N - k];
-          return C2<R>{v.n.x + v.s.y, v.s.x - v.n.y};
-        }
-        return C2<R>{R(0), R(0)};
+        const NS2<R>* e = fcl + (f0 + f) * NM;  // row (pair pl, field fl) = pl * FB + fl
+        const bool lo = k <= N / 2;
+        const int kk = lo ? k : N - k;
+        const bool in = kk <= T;
+        const NS2<R> v = e[in ? kk : 0];
+        const R ny = (k == 0) ? R(0) : v.n.y, sy = (k == 0) ? R(0) : v.s.y;
+        const C2<R> z = lo ? C2<R>{v.n.x - sy, ny + v.s.x} : C2<R>{v.n.x + sy, v.s.x - ny};
+        return in ? z : C2<R>{R(0), R(0)};
       };
       auto store = [&](int f, int k, C2<R> v) {
         const int gi = f0 + f, pl = gi / FB, fl = gi % FB;
@@ -393,11 +422,14 @@ template <int T, int MODE, bool FUSED, int V>
 struct FwdLds {
   using R = RealT<MODE>;
   using C = FwdCfg<T, MODE, V>;
-  static constexpr size_t F = sizeof(R) * TS<T>::NM * 2 * C::PB * 2 * C::FBF;
-  static constexpr size_t FBUF = FUSED ? sizeof(C2<R>) * FftBatch<TS<T>::NLON>::NF * TS<T>::NLON : 0;
+  // one (m, class) row of F: PB pairs x NCOL columns plus one pad of NCOL
+  // values, so that the rows that one wave reads start on different banks
+  static constexpr int RS = (C::PB + 1) * 2 * C::FBF;
+  static constexpr size_t F = sizeof(R) * TS<T>::NM * 2 * RS;
+  static constexpr size_t FBUF =
+      FUSED ? sizeof(C2<R>) * fft_buf_elems(FftBatch<TS<T>::NLON>::NF * TS<T>::NLON) : 0;
   static constexpr size_t TWS = FUSED ? sizeof(C2<R>) * TS<T>::NLON : 0;
-  static constexpr size_t MW = (TS<T>::NCSP + 15) / 16 * 16;
-  static constexpr size_t BYTES = F + FBUF + TWS + MW;
+  static constexpr size_t BYTES = F + FBUF + TWS;
 };
 
 template <int T, int MODE, int V>
@@ -422,7 +454,6 @@ __device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* 
   R* F = (R*)lds;
   C2<R>* fbuf = (C2<R>*)(lds + L::F);
   C2<R>* tws = (C2<R>*)(lds + L::F + L::FBUF);
-  uint8_t* mw = (uint8_t*)(lds + L::F + L::FBUF + L::TWS);
 
   const int tid = item_tid();
   const int ngroups = (a.nfield + FBF - 1) / FBF;
@@ -432,15 +463,13 @@ __device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* 
   if (mem >= a.nmem) return;
   const R* inm = (const R*)a.in + mem * a.in_stride;
 
-  for (int m = tid; m <= T; m += 256)
-    for (int n = m; n <= T; ++n) mw[w0_of(T, m) + n - m] = (uint8_t)m;
   if (FUSED) {
     const C2<R>* tw = (const C2<R>*)twp;
     for (int k = tid; k < N; k += 256) tws[k] = C2<R>{tw[k].x, -tw[k].y};
   }
   __syncthreads();
 
-  auto fidx = [](int m, int cls, int pl, int col) { return ((m * 2 + cls) * PB + pl) * NCOL + col; };
+  auto fidx = [](int m, int cls, int pl, int col) { return (m * 2 + cls) * L::RS + pl * NCOL + col; };
 
   if constexpr (FUSED) {
     for (int f0 = 0; f0 < PB * FBF; f0 += NF) {
@@ -453,14 +482,14 @@ __device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* 
         const R* src = inm + (long)(a.src0 + field) * NLAT * N;
         return C2<R>{src[(long)lat * N + k], src[(long)(NLAT - 1 - lat) * N + k]};
       };
-      auto store = [&](int f, int k, C2<R> v) { fbuf[f * N + k] = v; };
+      auto store = [&](int f, int k, C2<R> v) { fbuf[bix(f * N + k)] = v; };
       fft_run<N, -1, R>(fbuf, tws, nf, load, store);
       // separate the two real rows, scale by 1/N, fold into E/O
       const R sc = R(0.5) / R(N);
       for (int idx = tid; idx < nf * NM; idx += 256) {
         const int f = idx % nf, k = idx / nf;
-        const C2<R> zk = fbuf[f * N + k];
-        const C2<R> zn = fbuf[f * N + ((N - k) % N)];
+        const C2<R> zk = fbuf[bix(f * N + k)];
+        const C2<R> zn = fbuf[bix(f * N + ((N - k) % N))];
         const R cnx = (zk.x + zn.x) * sc, cny = (zk.y - zn.y) * sc;
         const R csx = (zk.y + zn.y) * sc, csy = (zn.x - zk.x) * sc;
         const int gi = f0 + f, pl = gi / FBF, fl = gi % FBF;
@@ -498,7 +527,7 @@ __device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* 
   const bool qtab = table_is_q(a.table);
   const R* tv = (const R*)tabv + (long)a.table * S::NPAIR * S::NCSP + (long)pb * PB * S::NCSP;
   for (int w = tid; w < S::NCSP; w += 256) {
-    const int m = mw[w];
+    const int m = m_of_w(T, w);
     const int n = m + (w - w0_of(T, m));
     const int cls = ((((n - m) & 1) == 0) != qtab) ? 0 : 1;
     const R* fr = &F[fidx(m, cls, 0, 0)];

@@ -16,6 +16,8 @@ struct exo_sht_plan_s {
   size_t scratch_bytes = 0;
   void* stage = nullptr;    // device staging of the host drop-in calls
   size_t stage_bytes = 0;
+  void* pk = nullptr;       // pre-split spectral operand of the split-BF16 inverse
+  size_t pk_bytes = 0;
   int nlev = 0;
   // tile variants per precision mode, chosen by exo_sht_autotune
   int inv_var[4] = {0, 0, 0, 0}, uv_var[4] = {0, 0, 0, 0}, fwd_var[4] = {0, 0, 0, 0};
@@ -182,6 +184,7 @@ extern "C" int exo_sht_plan_destroy(exo_sht_plan_t p) {
   free_tables(p->tb);
   if (p->scratch) (void)hipFree(p->scratch);
   if (p->stage) (void)hipFree(p->stage);
+  if (p->pk) (void)hipFree(p->pk);
   delete p;
   return EXO_OK;
 }
@@ -209,12 +212,28 @@ extern "C" int exo_sht_plan_tables(exo_sht_plan_t p, double* tables) {
 
 static bool mode_ok(int mode) { return mode >= EXO_F64 && mode <= EXO_BF16X3; }
 
-extern "C" int exo_sht_inverse(exo_sht_plan_t p, int mode, int table, int nmem, const void* sp,
-                               long sp_stride, int src0, void* gp, long gp_stride, int dst0,
-                               int nfield, void* stream) {
-  if (!p || !mode_ok(mode) || (table != EXO_QI && table != EXO_QJ) || nmem < 0 || nfield < 0 ||
-      (nmem > 0 && nfield > 0 && (!sp || !gp)))
-    return EXO_ERR_ARG;
+// MODE_BF16X3: pre-split the fp32 spectral fields [f0, f0 + nf) of every
+// member into the plan's pack buffer and point a at it
+static int pack_for(exo_sht_plan_t p, int mode, int nmem, const void* sp, long sps, int f0,
+                    int nf, hipStream_t s, InvArgs& a) {
+  a.bpk = nullptr;
+  a.bpk_stride = 0;
+  a.bpk_f0 = 0;
+  if (mode != MODE_BF16X3 || nmem == 0 || nf == 0) return EXO_OK;
+  int rc = grow(&p->pk, &p->pk_bytes, pack_bytes(p->tb, nmem, nf));
+  if (rc != EXO_OK) return rc;
+  if (launch_pack(p->tb, (const float*)sp, sps, f0, nf, nmem, (uint16_t*)p->pk, s) != 0)
+    return EXO_ERR_LAUNCH;
+  a.bpk = p->pk;
+  a.bpk_stride = (long)nf * pk_blocks(p->tb.ntru) * PK_BLK;
+  a.bpk_f0 = f0;
+  return EXO_OK;
+}
+
+// pk: an existing pack of the fields (round trip), or null to pack here
+static int inverse_impl(exo_sht_plan_t p, int mode, int table, int nmem, const void* sp,
+                        long sp_stride, int src0, void* gp, long gp_stride, int dst0, int nfield,
+                        hipStream_t s, const InvArgs* pk) {
   InvArgs a{};
   a.sp = sp;
   a.out = gp;
@@ -228,14 +247,20 @@ extern "C" int exo_sht_inverse(exo_sht_plan_t p, int mode, int table, int nmem, 
   a.nfield = nfield;
   a.plavor = p->tb.plavor;
   a.var = p->inv_var[mode];
-  return launch_inverse(p->tb, mode, true, a, (hipStream_t)stream);
+  if (pk) {
+    a.bpk = pk->bpk;
+    a.bpk_stride = pk->bpk_stride;
+    a.bpk_f0 = pk->bpk_f0;
+  } else {
+    const int rc = pack_for(p, mode, nmem, sp, sp_stride, src0, nfield, s, a);
+    if (rc != EXO_OK) return rc;
+  }
+  return launch_inverse(p->tb, mode, true, a, s);
 }
 
-extern "C" int exo_sht_inverse_uv(exo_sht_plan_t p, int mode, int nmem, const void* sp,
-                                  long sp_stride, int vor0, int div0, void* gp, long gp_stride,
-                                  int u0, int v0, int nlev, void* stream) {
-  if (!p || !mode_ok(mode) || nmem < 0 || nlev < 0 || (nmem > 0 && nlev > 0 && (!sp || !gp)))
-    return EXO_ERR_ARG;
+static int inverse_uv_impl(exo_sht_plan_t p, int mode, int nmem, const void* sp, long sp_stride,
+                           int vor0, int div0, void* gp, long gp_stride, int u0, int v0, int nlev,
+                           hipStream_t s, const InvArgs* pk) {
   InvArgs a{};
   a.sp = sp;
   a.out = gp;
@@ -250,7 +275,40 @@ extern "C" int exo_sht_inverse_uv(exo_sht_plan_t p, int mode, int nmem, const vo
   a.nlev = nlev;
   a.plavor = p->tb.plavor;
   a.var = p->uv_var[mode];
-  return launch_inverse(p->tb, mode, true, a, (hipStream_t)stream);
+  if (pk) {
+    a.bpk = pk->bpk;
+    a.bpk_stride = pk->bpk_stride;
+    a.bpk_f0 = pk->bpk_f0;
+  } else {
+    // one pack covering both runs of levels
+    const int lo = vor0 < div0 ? vor0 : div0;
+    const int hi = (vor0 > div0 ? vor0 : div0) + nlev;
+    const int rc = pack_for(p, mode, nmem, sp, sp_stride, lo, hi - lo, s, a);
+    if (rc != EXO_OK) return rc;
+  }
+  return launch_inverse(p->tb, mode, true, a, s);
+}
+
+extern "C" int exo_sht_inverse(exo_sht_plan_t p, int mode, int table, int nmem, const void* sp,
+                               long sp_stride, int src0, void* gp, long gp_stride, int dst0,
+                               int nfield, void* stream) {
+  if (!p || !mode_ok(mode) || (table != EXO_QI && table != EXO_QJ) || nmem < 0 || nfield < 0 ||
+      src0 < 0 || dst0 < 0 || (nmem > 0 && nfield > 0 && (!sp || !gp)))
+    return EXO_ERR_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
+  return inverse_impl(p, mode, table, nmem, sp, sp_stride, src0, gp, gp_stride, dst0, nfield,
+                      (hipStream_t)stream, nullptr);
+}
+
+extern "C" int exo_sht_inverse_uv(exo_sht_plan_t p, int mode, int nmem, const void* sp,
+                                  long sp_stride, int vor0, int div0, void* gp, long gp_stride,
+                                  int u0, int v0, int nlev, void* stream) {
+  if (!p || !mode_ok(mode) || nmem < 0 || nlev < 0 || vor0 < 0 || div0 < 0 ||
+      (nmem > 0 && nlev > 0 && (!sp || !gp)))
+    return EXO_ERR_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
+  return inverse_uv_impl(p, mode, nmem, sp, sp_stride, vor0, div0, gp, gp_stride, u0, v0, nlev,
+                         (hipStream_t)stream, nullptr);
 }
 
 static int direct_impl(exo_sht_plan_t p, int mode, bool fused, int nmem, const void* in,
@@ -288,20 +346,27 @@ extern "C" int exo_sht_direct(exo_sht_plan_t p, int mode, int nmem, const void* 
 extern "C" int exo_sht_roundtrip(exo_sht_plan_t p, int mode, int nmem, int nlev, const void* sp_in,
                                  void* gp, void* sp_out, void* stream) {
   if (!p || !mode_ok(mode) || nmem < 0 || nlev < 1) return EXO_ERR_ARG;
+  if (nmem > 0 && (!sp_in || !gp || !sp_out)) return EXO_ERR_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
   const Tables& t = p->tb;
+  hipStream_t s = (hipStream_t)stream;
   const int nfl = 3 * nlev + 1;
   const int ngp = nfl + 1 + 2 * nlev;
   const long sps = (long)nfl * 2 * t.ncsp;
   const long gps = (long)ngp * t.nlat * t.nlon;
-  int rc;
-  rc = exo_sht_inverse(p, mode, EXO_QI, nmem, sp_in, sps, 0, gp, gps, 0, nfl, stream);
+  // one pre-split of all nfl field-levels serves the three inverse launches
+  InvArgs pk{};
+  int rc = pack_for(p, mode, nmem, sp_in, sps, 0, nfl, s, pk);
   if (rc) return rc;
-  rc = exo_sht_inverse(p, mode, EXO_QJ, nmem, sp_in, sps, nfl - 1, gp, gps, nfl, 1, stream);
+  const InvArgs* pkp = (mode == MODE_BF16X3) ? &pk : nullptr;
+  rc = inverse_impl(p, mode, EXO_QI, nmem, sp_in, sps, 0, gp, gps, 0, nfl, s, pkp);
   if (rc) return rc;
-  rc = exo_sht_inverse_uv(p, mode, nmem, sp_in, sps, 0, nlev, gp, gps, nfl + 1, nfl + 1 + nlev,
-                          nlev, stream);
+  rc = inverse_impl(p, mode, EXO_QJ, nmem, sp_in, sps, nfl - 1, gp, gps, nfl, 1, s, pkp);
   if (rc) return rc;
-  return exo_sht_direct(p, mode, nmem, gp, gps, 0, sp_out, sps, 0, nfl, stream);
+  rc = inverse_uv_impl(p, mode, nmem, sp_in, sps, 0, nlev, gp, gps, nfl + 1, nfl + 1 + nlev, nlev,
+                       s, pkp);
+  if (rc) return rc;
+  return direct_impl(p, mode, true, nmem, gp, gps, 0, sp_out, sps, 0, nfl, s);
 }
 
 extern "C" int exo_sht_set_tiles(exo_sht_plan_t p, int mode, int inv_var, int uv_var,

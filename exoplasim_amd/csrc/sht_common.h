@@ -45,6 +45,45 @@ __host__ __device__ constexpr int w0_of(int ntru, int m) {
   return m * (ntru + 1) - m * (m - 1) / 2;
 }
 
+// zonal wavenumber of spectral index w: the root of w0_of(m) = w, then one
+// integer correction either way
+__device__ __forceinline__ int m_of_w(int ntru, int w) {
+  const int b = 2 * ntru + 3;
+  int m = (int)(0.5f * ((float)b - sqrtf((float)(b * b - 8 * w))));
+  m = m < 0 ? 0 : (m > ntru ? ntru : m);
+  if (m < ntru && w0_of(ntru, m + 1) <= w) ++m;
+  if (m > 0 && w0_of(ntru, m) > w) --m;
+  return m;
+}
+
+// Pre-split spectral operand of the split-BF16 inverse transform.  Per
+// field, for each zonal wavenumber m and each 32-wide block kb of total
+// wavenumbers n = m + 32 kb + i: [plane hi/mid/lo][re/im][32 n] bf16, zero
+// for n > NTRU.  pk_cum(x) = sum_{a=1..x} ceil(a/32), so the block of (m, kb)
+// starts at block pk_blkoff(m) + kb and a field holds pk_cum(NTRU+1) blocks.
+constexpr int PK_BLK = 3 * 2 * 32;  // bf16 values per (m, kb) block
+__host__ __device__ constexpr int pk_cum(int x) {
+  return 16 * (x / 32) * (x / 32 + 1) + (x % 32) * (x / 32 + 1);
+}
+__host__ __device__ constexpr int pk_blocks(int ntru) { return pk_cum(ntru + 1); }
+__host__ __device__ constexpr int pk_blkoff(int ntru, int m) {
+  return pk_cum(ntru + 1) - pk_cum(ntru + 1 - m);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2: block b runs on XCD b % 8.  This renumbering gives each XCD one
+// contiguous range of logical blocks, so blocks with neighbouring logical
+// indices (the latitude-pair blocks of one member's field group, which read
+// the same spectral coefficients) share an L2 instead of fetching the same
+// data from HBM once per XCD.  A bijection on [0, nblk) whatever the
+// dispatch order.
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+  constexpr int NX = 8;
+  const int q = nblk / NX, r = nblk % NX;
+  const int x = b % NX, i = b / NX;
+  return x * q + (x < r ? x : r) + i;
+}
+
 template <typename R>
 struct C2 {
   R x, y;
@@ -140,6 +179,12 @@ struct FftBatch {
   static constexpr int NF = (N == 64) ? 32 : (N == 128) ? 16 : (N == 192) ? 10 : 5;
 };
 
+// FFT work rows in LDS hold one pad element after every 32 complex values:
+// the first pass writes with a stride of 8 complex (64 bytes), which without
+// the pad would put 16 lanes on each bank.
+__host__ __device__ constexpr int fft_buf_elems(int n) { return n + (n >> 5); }
+__device__ __forceinline__ int bix(int i) { return i + (i >> 5); }
+
 // One Stockham autosort pass (radix R, stride NS) over NF FFTs of length
 // N held in LDS.  Pass 0 reads through `load(f, k)`, the last pass writes
 // through `store(f, k, v)`; the other passes go through `buf` (NF*N
@@ -175,7 +220,7 @@ __device__ __forceinline__ void fft_pass(C2<F>* buf, const C2<F>* tw, int nf,
         if (FIRST)
           v[t][r] = load(f, j + r * NB);
         else
-          v[t][r] = buf[f * N + j + r * NB];
+          v[t][r] = buf[bix(f * N + j + r * NB)];
       }
     }
   }
@@ -197,7 +242,7 @@ __device__ __forceinline__ void fft_pass(C2<F>* buf, const C2<F>* tw, int nf,
         if (LAST)
           store(f, idxd + r * NS, v[t][r]);
         else
-          buf[f * N + idxd + r * NS] = v[t][r];
+          buf[bix(f * N + idxd + r * NS)] = v[t][r];
       }
     }
   }

@@ -204,6 +204,7 @@ struct exo_sht_ens_s {
   int ngroup = 0;
   exo_sht_ens_group g[ENS_MAXGROUP];
   exo_sht_plan_t plan[ENS_MAXGROUP] = {};  // one per group (own tables)
+  uint16_t* pk[ENS_MAXGROUP] = {};         // BF16X3: pre-split spectral input per group
   int inv_var[ENS_MAXGROUP] = {}, fwd_var[ENS_MAXGROUP] = {};
   // device state
   EnsInvJob* dinv = nullptr;
@@ -260,6 +261,11 @@ static int ens_prepare(exo_sht_ens_s* e) {
       j.a.nmem = g.nmem;
       j.a.plavor = tb.plavor;
       j.a.var = e->inv_var[k];
+      if (e->mode == MODE_BF16X3) {
+        j.a.bpk = e->pk[k];
+        j.a.bpk_stride = (long)nfl * pk_blocks(tb.ntru) * PK_BLK;
+        j.a.bpk_f0 = 0;
+      }
       int t0 = TQI;
       if (kind == 2) {
         j.a.kind = INV_UV;
@@ -388,12 +394,27 @@ static int ens_launch(exo_sht_ens_s* e, bool inv, hipStream_t s) {
 static int ens_launch_inv(exo_sht_ens_s* e, hipStream_t s) { return ens_launch(e, true, s); }
 static int ens_launch_fwd(exo_sht_ens_s* e, hipStream_t s) { return ens_launch(e, false, s); }
 
-// the whole ensemble round trip: reset counters, all inverse items, all
-// direct items
+// BF16X3: the pre-split spectral input of every group
+static int ens_pack(exo_sht_ens_s* e, hipStream_t s) {
+  if (e->mode != MODE_BF16X3) return EXO_OK;
+  for (int k = 0; k < e->ngroup; ++k) {
+    const exo_sht_ens_group& g = e->g[k];
+    const Tables& tb = plan_tables(e->plan[k]);
+    const int nfl = 3 * g.nlev + 1;
+    if (launch_pack(tb, (const float*)g.sp_in, (long)nfl * 2 * tb.ncsp, 0, nfl, g.nmem, e->pk[k],
+                    s) != 0)
+      return EXO_ERR_LAUNCH;
+  }
+  return EXO_OK;
+}
+
+// the whole ensemble round trip: reset counters, pre-split the inputs, all
+// inverse items, all direct items
 static int ens_enqueue(exo_sht_ens_s* e, hipStream_t s) {
   k_ens_reset<<<1, 64, 0, s>>>(e->counter);
   ENS_HIPCK(hipGetLastError());
-  int rc = ens_launch_inv(e, s);
+  int rc = ens_pack(e, s);
+  if (rc == EXO_OK) rc = ens_launch_inv(e, s);
   if (rc == EXO_OK) rc = ens_launch_fwd(e, s);
   return rc;
 }
@@ -427,6 +448,8 @@ static void ens_free(exo_sht_ens_s* e) {
   if (e->dinv) (void)hipFree(e->dinv);
   if (e->dfwd) (void)hipFree(e->dfwd);
   if (e->dstart) (void)hipFree(e->dstart);
+  for (int k = 0; k < e->ngroup; ++k)
+    if (e->pk[k]) (void)hipFree(e->pk[k]);
   if (e->counter) (void)hipFree(e->counter);
   for (int k = 0; k < e->ngroup; ++k)
     if (e->plan[k]) exo_sht_plan_destroy(e->plan[k]);
@@ -453,6 +476,10 @@ extern "C" int exo_sht_ens_create(exo_sht_ens_t* ens, int mode, int ngroup,
     e->g[k] = groups[k];
     rc = exo_sht_plan_create(&e->plan[k], groups[k].ntru, nullptr, nullptr, nullptr, nullptr,
                              1.63299310207);
+    if (rc == EXO_OK && mode == EXO_BF16X3 && groups[k].nmem > 0 &&
+        hipMalloc(&e->pk[k], pack_bytes(plan_tables(e->plan[k]), groups[k].nmem,
+                                        3 * groups[k].nlev + 1)) != hipSuccess)
+      rc = EXO_ERR_HIP;
   }
   if (rc == EXO_OK &&
       (hipMalloc(&e->dinv, sizeof(EnsInvJob) * ENS_MAXJOB) != hipSuccess ||
@@ -565,6 +592,8 @@ extern "C" int exo_sht_ens_autotune(exo_sht_ens_t e, void* stream, int* vars) {
   int rc = EXO_OK;
   float best[2] = {1e30f, 1e30f};
   int pick[2] = {0, 0};
+  int rc0 = ens_pack(e, s);  // the inverse timings read the pre-split inputs
+  if (rc0 != EXO_OK) rc = rc0;
   for (int combo = 0; combo < 8 && rc == EXO_OK; ++combo) {
     (void)hipStreamSynchronize(s);
     for (int k = 0; k < e->ngroup; ++k) {

@@ -54,6 +54,11 @@ struct InvArgs {
   int vor0, div0, u0, v0, nlev;         // INV_UV
   double plavor;
   int var;          // tile variant (0 default, 1 half the fields per block)
+  // MODE_BF16X3: the pre-split spectral operand (launch_pack) of fields
+  // [bpk_f0, ...), bpk_stride bf16 values per member
+  const void* bpk;
+  long bpk_stride;
+  int bpk_f0;
 };
 
 struct FwdArgs {
@@ -87,6 +92,14 @@ int launch_forward(const Tables& T, int mode, bool fused, const FwdArgs& a, void
                    size_t scratch_bytes, hipStream_t s);
 size_t forward_scratch_bytes(const Tables& T, int mode, int nmem, int nfield);
 int launch_fft(const Tables& T, int mode, int dir, void* a, int n, long lot, hipStream_t s);
+// pre-split copy (PK layout, sht_common.h) of fp32 spectral fields [f0, f0+nf)
+// of nmem members for the split-BF16 inverse; out holds nmem * nf *
+// pk_blocks(ntru) * PK_BLK bf16 values
+int launch_pack(const Tables& T, const float* sp, long sp_stride, int f0, int nf, int nmem,
+                uint16_t* out, hipStream_t s);
+inline size_t pack_bytes(const Tables& T, int nmem, int nf) {
+  return sizeof(uint16_t) * (size_t)nmem * nf * pk_blocks(T.ntru) * PK_BLK;
+}
 
 // the device tables of a plan (defined with the plan object)
 const Tables& plan_tables(const exo_sht_plan_s* p);

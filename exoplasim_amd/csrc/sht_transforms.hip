@@ -30,7 +30,7 @@ namespace exo {
 template <int N, typename R, int DIR>
 __global__ __launch_bounds__(256) void k_fft_rows(R* a, long lot, const C2<R>* tw) {
   constexpr int NF = FftBatch<N>::NF;
-  __shared__ C2<R> fbuf[NF * N];
+  __shared__ C2<R> fbuf[fft_buf_elems(NF * N)];
   __shared__ C2<R> tws[N];
   for (int k = threadIdx.x; k < N; k += 256)
     tws[k] = DIR > 0 ? tw[k] : C2<R>{tw[k].x, -tw[k].y};
@@ -71,13 +71,13 @@ __global__ __launch_bounds__(256) void k_fft_rows(R* a, long lot, const C2<R>* t
       const R* x1 = row(f, 1);
       return C2<R>{x0 ? x0[k] : R(0), x1 ? x1[k] : R(0)};
     };
-    auto store = [&](int f, int k, C2<R> v) { fbuf[f * N + k] = v; };
+    auto store = [&](int f, int k, C2<R> v) { fbuf[bix(f * N + k)] = v; };
     fft_run<N, -1, R>(fbuf, tws, nf, load, store);
     const R sc = R(0.5) / R(N);
     for (int idx = threadIdx.x; idx < nf * (N / 2); idx += 256) {
       const int f = idx / (N / 2), k = idx % (N / 2);
-      const C2<R> zk = fbuf[f * N + k];
-      const C2<R> zn = fbuf[f * N + ((N - k) % N)];
+      const C2<R> zk = fbuf[bix(f * N + k)];
+      const C2<R> zn = fbuf[bix(f * N + ((N - k) % N))];
       R* x0 = row(f, 0);
       R* x1 = row(f, 1);
       if (x0) {
@@ -133,6 +133,67 @@ static int fft_t(const void* tw, int dir, R* a, long lot, hipStream_t s) {
   else
     k_fft_rows<N, R, -1><<<(unsigned)nblk, 256, 0, s>>>(a, lot, (const C2<R>*)tw);
   return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+// ---------------------------------------------------------------------------
+// pre-split spectral operand of the split-BF16 inverse (layout in
+// sht_common.h): one thread per 8 consecutive n of one (member, field, m, kb,
+// re/im); the three planes are written as 16-byte chunks, eight threads to a
+// contiguous 128-byte plane row
+// ---------------------------------------------------------------------------
+template <int T>
+__global__ __launch_bounds__(256) void k_pack_sp(const float* sp, long sp_stride, int f0, int nf,
+                                                 int nmem, uint16_t* out) {
+  using S = TS<T>;
+  constexpr int NB = pk_blocks(T);
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)nmem * nf * NB * 8) return;
+  const int ri = (int)(i >> 2) & 1, q = (int)i & 3;
+  const long bi = i >> 3;  // (member, field, block)
+  const int blk = (int)(bi % NB);
+  const long fm = bi / NB;
+  const int field = (int)(fm % nf), mem = (int)(fm / nf);
+  int lo = 0, hi = T;  // the largest m whose first block is <= blk
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pk_blkoff(T, mid) <= blk)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int m = lo, kb = blk - pk_blkoff(T, m);
+  const float* src = sp + mem * sp_stride + (long)(f0 + field) * S::NRSP + 2 * w0_of(T, m) + ri;
+  BF8 h, md, l;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int dn = kb * 32 + q * 8 + j;  // n - m
+    const float x = (m + dn <= T) ? src[2 * dn] : 0.f;
+    split3(x, h.s[j], md.s[j], l.s[j]);
+  }
+  uint16_t* dst = out + bi * PK_BLK + ri * 32 + q * 8;
+  *(uint4*)dst = h.q;
+  *(uint4*)(dst + 64) = md.q;
+  *(uint4*)(dst + 128) = l.q;
+}
+
+template <int T>
+static int pack_t(const float* sp, long sps, int f0, int nf, int nmem, uint16_t* out,
+                  hipStream_t s) {
+  const long total = (long)nmem * nf * pk_blocks(T) * 8;
+  if (total <= 0) return 0;
+  k_pack_sp<T><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(sp, sps, f0, nf, nmem, out);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_pack(const Tables& tb, const float* sp, long sp_stride, int f0, int nf, int nmem,
+                uint16_t* out, hipStream_t s) {
+  switch (tb.ntru) {
+    case 21: return pack_t<21>(sp, sp_stride, f0, nf, nmem, out, s);
+    case 42: return pack_t<42>(sp, sp_stride, f0, nf, nmem, out, s);
+    case 63: return pack_t<63>(sp, sp_stride, f0, nf, nmem, out, s);
+    case 127: return pack_t<127>(sp, sp_stride, f0, nf, nmem, out, s);
+    default: return -2;
+  }
 }
 
 int launch_fft(const Tables& tb, int mode, int dir, void* a, int n, long lot, hipStream_t s) {

@@ -12,14 +12,15 @@ template <int T, int MODE, int KIND, bool FUSED, int V>
 __global__ __launch_bounds__(256) void k_inv(InvArgs a, const void* tabA0, const void* tabA1,
                                              long plane, const void* tabv, const void* twp) {
   __shared__ __align__(16) char lds[InvLds<T, MODE, FUSED, V>::BYTES];
-  inv_body<T, MODE, KIND, FUSED, V>(a, blockIdx.x, tabA0, tabA1, plane, tabv, twp, lds);
+  inv_body<T, MODE, KIND, FUSED, V>(a, xcd_block(blockIdx.x, gridDim.x), tabA0, tabA1, plane,
+                                    tabv, twp, lds);
 }
 
 template <int T, int MODE, bool FUSED, int V>
 __global__ __launch_bounds__(256) void k_fwd(FwdArgs a, const void* tabv, const void* twp,
                                              void* scratch) {
   __shared__ __align__(16) char lds[FwdLds<T, MODE, FUSED, V>::BYTES];
-  fwd_body<T, MODE, FUSED, V>(a, blockIdx.x, tabv, twp, scratch, lds);
+  fwd_body<T, MODE, FUSED, V>(a, xcd_block(blockIdx.x, gridDim.x), tabv, twp, scratch, lds);
 }
 
 // sum of the per-pair-block partial direct transforms in a fixed order

@@ -327,21 +327,30 @@ class LwEmulator:
         net = _Net(self.nlev, (c_void_p * 4)(*[_ptr(w) for w in ws]),
                    (c_void_p * 4)(*[_ptr(b) for b in bs]), _ptr(ym), _ptr(ys))
         self._net = (net, ws, bs, ym, ys)  # keeps the buffers alive
+        self._net_ref = ctypes.byref(net)
+        self._emulate = L.exo_lw_emulate
 
-    def predict(self, cols):
+    def predict(self, cols, out=None):
         """heating (nlev, ncol) K/day and OLR (ncol) of device columns, in
-        one kernel (exo_lw_emulate)."""
+        one kernel (exo_lw_emulate).  out=(heat, olr) reuses fp32 buffers."""
         import torch
 
         if self._net is None:
             self._freeze()
-        c = {k: cols[k].float().contiguous() for k in ("dt", "dq", "dp", "dqco2")}
-        ncol = c["dp"].shape[0]
-        heat = torch.empty((self.nlev, ncol), dtype=torch.float32, device=c["dp"].device)
-        olr = torch.empty(ncol, dtype=torch.float32, device=c["dp"].device)
-        _chk(_proto().exo_lw_emulate(ctypes.byref(self._net[0]), ncol, _ptr(c["dt"]),
-                                     _ptr(c["dq"]), _ptr(c["dp"]), _ptr(c["dqco2"]), _ptr(heat),
-                                     _ptr(olr), _stream()), "exo_lw_emulate")
+
+        def f32(t):
+            return t if (t.dtype == torch.float32 and t.is_contiguous()) else t.float().contiguous()
+
+        dt, dq, dp, co2 = f32(cols["dt"]), f32(cols["dq"]), f32(cols["dp"]), f32(cols["dqco2"])
+        ncol = dp.shape[0]
+        if out is None:
+            heat = torch.empty((self.nlev, ncol), dtype=torch.float32, device=dp.device)
+            olr = torch.empty(ncol, dtype=torch.float32, device=dp.device)
+        else:
+            heat, olr = out
+        _chk(self._emulate(self._net_ref, ncol, dt.data_ptr(), dq.data_ptr(), dp.data_ptr(),
+                           co2.data_ptr(), heat.data_ptr(), olr.data_ptr(), _stream()),
+             "exo_lw_emulate")
         return heat, olr
 
     def predict_layers(self, cols):
