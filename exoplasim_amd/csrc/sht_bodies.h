@@ -226,43 +226,55 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
         }
       }
     };
-    auto mul = [&](const BFrag& br, const uint4* a, f32x4 acc) -> f32x4 {
+    // two independent accumulation chains per term (the small cross terms
+    // and the large ones), so that consecutive MFMAs do not all wait for the
+    // previous result
+    auto mul = [&](const BFrag& br, const uint4* a, f32x4& acc, f32x4& acc2) {
       if constexpr (MODE == MODE_BF16X3) {
         BF8 ah, am, al;
         ah.q = a[0];
         am.q = a[1];
         al.q = a[2];
         acc = mfma_bf16(ah, br.p[2], acc);
+        acc2 = mfma_bf16(am, br.p[0], acc2);
         acc = mfma_bf16(am, br.p[1], acc);
+        acc2 = mfma_bf16(ah, br.p[1], acc2);
         acc = mfma_bf16(al, br.p[0], acc);
-        acc = mfma_bf16(ah, br.p[1], acc);
-        acc = mfma_bf16(am, br.p[0], acc);
-        acc = mfma_bf16(ah, br.p[0], acc);
+        acc2 = mfma_bf16(ah, br.p[0], acc2);
       } else {
         const float* af = (const float*)a;
 #pragma unroll
-        for (int j = 0; j < 8; ++j)
+        for (int j = 0; j < 8; j += 2) {
           acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[j], br.f[j], acc, 0, 0, 0);
+          acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(af[j + 1], br.f[j + 1], acc2, 0, 0, 0);
+        }
       }
-      return acc;
     };
-    BFrag br, br2;
-    uint4 a0[3], a1[3], a02[3], a12[3];
-    int m = wv, kb = 0;
-    if (m <= T) fetch(m, 0, br, a0, a1);
-    f32x4 acc1 = {0.f, 0.f, 0.f, 0.f}, acc2 = {0.f, 0.f, 0.f, 0.f};
-    while (m <= T) {
-      const int kbn = (NM - m + 31) / 32;
-      int m2 = m, kb2 = kb + 1;
-      if (kb2 >= kbn) {
-        m2 = m + 4;
-        kb2 = 0;
-      }
-      if (m2 <= T) fetch(m2, kb2, br2, a02, a12);
-      acc1 = mul(br, a0, acc1);
-      if (NTERM == 2) acc2 = mul(br, a1, acc2);
-      if (kb2 == 0) {
-        // all n of this m are summed: combine the terms, fold N/S, store
+    // operands of one work item (m, kb)
+    struct Frag {
+      BFrag b;
+      uint4 a0[3], a1[3];
+    };
+    struct Item {
+      int m, kb;
+    };
+    auto next_item = [&](Item it) -> Item {
+      const int kbn = (NM - it.m + 31) / 32;
+      return (it.kb + 1 < kbn) ? Item{it.m, it.kb + 1} : Item{it.m + 4, 0};
+    };
+    const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
+    f32x4 acc1 = zero4, acc1b = zero4, acc2 = zero4, acc2b = zero4;
+    // multiply item `cur` (operands in `use`) after starting the loads of
+    // item `ahead` (two items ahead) into `into`; at the last n block of an
+    // m, combine the terms, fold N/S and store the Fourier coefficients
+    auto step = [&](Item cur, Frag& use, Frag& into, Item ahead) {
+      if (ahead.m <= T) fetch(ahead.m, ahead.kb, into.b, into.a0, into.a1);
+      mul(use.b, use.a0, acc1, acc1b);
+      if (NTERM == 2) mul(use.b, use.a1, acc2, acc2b);
+      if (next_item(cur).kb == 0) {
+        const int m = cur.m;
+        acc1 += acc1b;
+        if (NTERM == 2) acc2 += acc2b;
         f32x4 out = acc1;
         if (KIND == INV_UV) {
           // u = qv.zeta + qu.(-i D), v = qu.(-i zeta) - qv.D: the qu product of
@@ -295,17 +307,31 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
             }
           }
         }
-        acc1 = f32x4{0.f, 0.f, 0.f, 0.f};
-        acc2 = f32x4{0.f, 0.f, 0.f, 0.f};
+        acc1 = acc1b = acc2 = acc2b = zero4;
       }
-      m = m2;
-      kb = kb2;
-      br = br2;
-#pragma unroll
-      for (int p = 0; p < 3; ++p) {
-        a0[p] = a02[p];
-        a1[p] = a12[p];
-      }
+    };
+    // three operand buffers in rotation, unrolled so that no buffer is
+    // copied: two items are in flight while one is multiplied
+    Frag f0, f1, f2;
+    Item i0 = {wv, 0};
+    Item i1 = next_item(i0), i2 = next_item(i1);
+    if (i0.m <= T) fetch(i0.m, i0.kb, f0.b, f0.a0, f0.a1);
+    if (i1.m <= T) fetch(i1.m, i1.kb, f1.b, f1.a0, f1.a1);
+    while (i0.m <= T) {
+      step(i0, f0, f2, i2);
+      i0 = i1;
+      i1 = i2;
+      i2 = next_item(i2);
+      if (i0.m > T) break;
+      step(i0, f1, f0, i2);
+      i0 = i1;
+      i1 = i2;
+      i2 = next_item(i2);
+      if (i0.m > T) break;
+      step(i0, f2, f1, i2);
+      i0 = i1;
+      i1 = i2;
+      i2 = next_item(i2);
     }
   } else {
     // vector-ALU Legendre: one (pair, column, m) sum per thread
