@@ -311,15 +311,12 @@ def main():
             plan.roundtrip(mode, sp_in, gp, sp_out, NLEV)
 
         t = time_loop(rt, args.steps, args.warmup)
-        # per-launch timing of the four kernels of a round trip
+        # per-call timing of the two halves of a round trip
         legs = {
-            "inverse_qi": (lambda: plan.inverse(mode, sp_in, gp, nfl, 0, 0, "qi"),
-                           nmem * nfl * (nrsp + nhor) * es),
-            "inverse_qj": (lambda: plan.inverse(mode, sp_in, gp, 1, nfl - 1, nfl, "qj"),
-                           nmem * (nrsp + nhor) * es),
-            "inverse_uv": (lambda: plan.inverse_uv(mode, sp_in, gp, NLEV, 0, NLEV, nfl + 1,
-                                                   nfl + 1 + NLEV),
-                           nmem * NLEV * 2 * (nrsp + nhor) * es),
+            # pre-split (split-BF16) + sp2fc, sp2fcdmu, dv2uv fused with fc2gp,
+            # one launch: each transformed field-level read and written once
+            "inverse_all": (lambda: plan.inverse_all(mode, sp_in, gp, NLEV),
+                            nmem * (nfl + 1 + 2 * NLEV) * (nrsp + nhor) * es),
             "direct_qc": (lambda: plan.direct(mode, gp, sp_out, nfl, 0, 0),
                           nmem * nfl * (nhor + nrsp) * es),
         }

@@ -50,6 +50,7 @@ PROTOTYPES = {
                                c_long, c_int, c_int, c_void_p]),
     "exo_sht_roundtrip": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p,
                                   c_void_p, c_void_p]),
+    "exo_sht_inverse_all": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "exo_sht_set_tiles": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     "exo_sht_get_tiles": (c_int, [c_void_p, c_int, POINTER(c_int)]),
     "exo_sht_autotune": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, POINTER(c_int)]),

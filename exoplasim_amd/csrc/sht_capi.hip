@@ -343,6 +343,9 @@ extern "C" int exo_sht_direct(exo_sht_plan_t p, int mode, int nmem, const void* 
                      (hipStream_t)stream);
 }
 
+static int inverse_all_impl(exo_sht_plan_t p, int mode, int nmem, int nlev, const void* sp_in,
+                            void* gp, hipStream_t s);
+
 extern "C" int exo_sht_roundtrip(exo_sht_plan_t p, int mode, int nmem, int nlev, const void* sp_in,
                                  void* gp, void* sp_out, void* stream) {
   if (!p || !mode_ok(mode) || nmem < 0 || nlev < 1) return EXO_ERR_ARG;
@@ -354,17 +357,9 @@ extern "C" int exo_sht_roundtrip(exo_sht_plan_t p, int mode, int nmem, int nlev,
   const int ngp = nfl + 1 + 2 * nlev;
   const long sps = (long)nfl * 2 * t.ncsp;
   const long gps = (long)ngp * t.nlat * t.nlon;
-  // one pre-split of all nfl field-levels serves the three inverse launches
-  InvArgs pk{};
-  int rc = pack_for(p, mode, nmem, sp_in, sps, 0, nfl, s, pk);
-  if (rc) return rc;
-  const InvArgs* pkp = (mode == MODE_BF16X3) ? &pk : nullptr;
-  rc = inverse_impl(p, mode, EXO_QI, nmem, sp_in, sps, 0, gp, gps, 0, nfl, s, pkp);
-  if (rc) return rc;
-  rc = inverse_impl(p, mode, EXO_QJ, nmem, sp_in, sps, nfl - 1, gp, gps, nfl, 1, s, pkp);
-  if (rc) return rc;
-  rc = inverse_uv_impl(p, mode, nmem, sp_in, sps, 0, nlev, gp, gps, nfl + 1, nfl + 1 + nlev, nlev,
-                       s, pkp);
+  // one pre-split of all nfl field-levels serves the three inverse transforms,
+  // which run as one launch
+  const int rc = inverse_all_impl(p, mode, nmem, nlev, sp_in, gp, s);
   if (rc) return rc;
   return direct_impl(p, mode, true, nmem, gp, gps, 0, sp_out, sps, 0, nfl, s);
 }
@@ -386,6 +381,54 @@ extern "C" int exo_sht_get_tiles(exo_sht_plan_t p, int mode, int* vars) {
   vars[1] = p->uv_var[mode];
   vars[2] = p->fwd_var[mode];
   return EXO_OK;
+}
+
+// the inverse half of exo_sht_roundtrip: pre-split (split-BF16) and the
+// three inverse transforms in one launch
+static int inverse_all_impl(exo_sht_plan_t p, int mode, int nmem, int nlev, const void* sp_in,
+                            void* gp, hipStream_t s) {
+  const Tables& t = p->tb;
+  const int nfl = 3 * nlev + 1;
+  const long sps = (long)nfl * 2 * t.ncsp;
+  const long gps = (long)(nfl + 1 + 2 * nlev) * t.nlat * t.nlon;
+  InvArgs pk{};
+  int rc = pack_for(p, mode, nmem, sp_in, sps, 0, nfl, s, pk);
+  if (rc) return rc;
+  InvArgs aq{};
+  aq.sp = sp_in;
+  aq.out = gp;
+  aq.sp_stride = sps;
+  aq.out_stride = gps;
+  aq.nmem = nmem;
+  aq.plavor = t.plavor;
+  aq.var = p->inv_var[mode];
+  aq.bpk = pk.bpk;
+  aq.bpk_stride = pk.bpk_stride;
+  aq.bpk_f0 = pk.bpk_f0;
+  InvArgs aj = aq, au = aq;
+  aq.kind = INV_SCALAR;
+  aq.table = TQI;
+  aq.nfield = nfl;
+  aj.kind = INV_SCALAR;
+  aj.table = TQJ;
+  aj.src0 = nfl - 1;
+  aj.dst0 = nfl;
+  aj.nfield = 1;
+  au.kind = INV_UV;
+  au.div0 = nlev;
+  au.u0 = nfl + 1;
+  au.v0 = nfl + 1 + nlev;
+  au.nlev = nlev;
+  rc = launch_inverse_rt(t, mode, aq, aj, au, s);
+  return rc == 0 ? EXO_OK : EXO_ERR_LAUNCH;
+}
+
+extern "C" int exo_sht_inverse_all(exo_sht_plan_t p, int mode, int nmem, int nlev,
+                                   const void* sp_in, void* gp, void* stream) {
+  if (!p || !mode_ok(mode) || nmem < 0 || nlev < 1) return EXO_ERR_ARG;
+  if (nmem > 0 && (!sp_in || !gp)) return EXO_ERR_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
+  return inverse_all_impl(p, mode, nmem, nlev, sp_in, gp, (hipStream_t)stream);
 }
 
 // time each tile variant of the three fused kernels on a round trip of
@@ -416,7 +459,7 @@ extern "C" int exo_sht_autotune(exo_sht_plan_t p, int mode, int nmem, int nlev, 
       for (int rep = 0; rep < 4 && rc == EXO_OK; ++rep) {
         (void)hipEventRecord(e0, s);
         if (k == 0)
-          rc = exo_sht_inverse(p, mode, EXO_QI, nmem, sp, sps, 0, gp, gps, 0, nfl, s);
+          rc = exo_sht_inverse_all(p, mode, nmem, nlev, sp, gp, s);
         else if (k == 1)
           rc = exo_sht_inverse_uv(p, mode, nmem, sp, sps, 0, nlev, gp, gps, nfl + 1,
                                   nfl + 1 + nlev, nlev, s);

@@ -165,13 +165,48 @@ struct Dft<8, S, F> {
   }
 };
 
+// 16 = 4 x 4: DFT4 over n1 of v[4 n1 + n2], twiddle w16^(n2 k1), DFT4 over
+// n2 into X[k1 + 4 k2]
+template <int S, typename F>
+struct Dft<16, S, F> {
+  __device__ __forceinline__ static void run(C2<F>* v) {
+    const F c1 = F(0.92387953251128675613), s1 = F(0.38268343236508977173);
+    const F r = F(0.70710678118654752440);
+    C2<F> t[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      C2<F> a[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+      Dft<4, S, F>::run(a);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) t[n2][k1] = a[k1];
+    }
+    // w16^m = (cos, S sin)(2 pi m / 16) for m = n2 k1
+    t[1][1] = cmul(t[1][1], C2<F>{c1, F(S) * s1});   // m = 1
+    t[1][2] = cmul(t[1][2], C2<F>{r, F(S) * r});     // m = 2
+    t[1][3] = cmul(t[1][3], C2<F>{s1, F(S) * c1});   // m = 3
+    t[2][1] = cmul(t[2][1], C2<F>{r, F(S) * r});     // m = 2
+    t[2][2] = cmul_si<S>(t[2][2]);                   // m = 4
+    t[2][3] = cmul(t[2][3], C2<F>{-r, F(S) * r});    // m = 6
+    t[3][1] = cmul(t[3][1], C2<F>{s1, F(S) * c1});   // m = 3
+    t[3][2] = cmul(t[3][2], C2<F>{-r, F(S) * r});    // m = 6
+    t[3][3] = cmul(t[3][3], C2<F>{-c1, -F(S) * s1}); // m = 9
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      C2<F> b[4] = {t[0][k1], t[1][k1], t[2][k1], t[3][k1]};
+      Dft<4, S, F>::run(b);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = b[k2];
+    }
+  }
+};
+
 // Radix plan of the row lengths used by the supported truncations.
 template <int N>
 struct FftPlan;
 template <> struct FftPlan<64>  { static constexpr int NP = 2; static constexpr int R[4] = {8, 8, 1, 1}; };
-template <> struct FftPlan<128> { static constexpr int NP = 3; static constexpr int R[4] = {8, 4, 4, 1}; };
+template <> struct FftPlan<128> { static constexpr int NP = 2; static constexpr int R[4] = {16, 8, 1, 1}; };
 template <> struct FftPlan<192> { static constexpr int NP = 3; static constexpr int R[4] = {8, 8, 3, 1}; };
-template <> struct FftPlan<384> { static constexpr int NP = 4; static constexpr int R[4] = {8, 4, 4, 3}; };
+template <> struct FftPlan<384> { static constexpr int NP = 3; static constexpr int R[4] = {16, 8, 3, 1}; };
 
 // FFTs in flight per pass within a 256-thread block (about 2048 points)
 template <int N>
@@ -180,8 +215,8 @@ struct FftBatch {
 };
 
 // FFT work rows in LDS hold one pad element after every 32 complex values:
-// the first pass writes with a stride of 8 complex (64 bytes), which without
-// the pad would put 16 lanes on each bank.
+// the first pass writes with a stride of 8 or 16 complex (64 or 128 bytes),
+// which without the pad would put 16 or 32 lanes on each bank.
 __host__ __device__ constexpr int fft_buf_elems(int n) { return n + (n >> 5); }
 __device__ __forceinline__ int bix(int i) { return i + (i >> 5); }
 

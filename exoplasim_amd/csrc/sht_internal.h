@@ -88,6 +88,10 @@ int launch_dirx(const Tables& T, int mode, const DirxArgs& a, void* scratch, siz
 size_t dirx_scratch_bytes(const Tables& T, int mode, int nlev);
 
 int launch_inverse(const Tables& T, int mode, bool fused, const InvArgs& a, hipStream_t s);
+// the sp2fc (aq), sp2fcdmu (aj) and dv2uv (au) transforms of a round trip,
+// fused with fc2gp, in one launch; tile variant aq.var
+int launch_inverse_rt(const Tables& T, int mode, const InvArgs& aq, const InvArgs& aj,
+                      const InvArgs& au, hipStream_t s);
 int launch_forward(const Tables& T, int mode, bool fused, const FwdArgs& a, void* scratch,
                    size_t scratch_bytes, hipStream_t s);
 size_t forward_scratch_bytes(const Tables& T, int mode, int nmem, int nfield);

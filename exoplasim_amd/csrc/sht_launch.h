@@ -8,6 +8,9 @@ namespace exo {
 template <int MODE>
 int inv_mode(const Tables& tb, bool fused, const InvArgs& a, hipStream_t s);
 template <int MODE>
+int inv_rt_mode(const Tables& tb, const InvArgs& aq, const InvArgs& aj, const InvArgs& au,
+                hipStream_t s);
+template <int MODE>
 int fwd_mode(const Tables& tb, bool fused, const FwdArgs& a, void* scr, size_t sb, hipStream_t s);
 template <int MODE>
 size_t scr_mode(const Tables& tb, int nmem, int nfield);

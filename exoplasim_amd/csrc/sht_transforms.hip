@@ -102,6 +102,17 @@ int launch_inverse(const Tables& tb, int mode, bool fused, const InvArgs& a, hip
   }
 }
 
+int launch_inverse_rt(const Tables& tb, int mode, const InvArgs& aq, const InvArgs& aj,
+                      const InvArgs& au, hipStream_t s) {
+  switch (mode) {
+    case MODE_F64: return inv_rt_mode<MODE_F64>(tb, aq, aj, au, s);
+    case MODE_F32: return inv_rt_mode<MODE_F32>(tb, aq, aj, au, s);
+    case MODE_F32MFMA: return inv_rt_mode<MODE_F32MFMA>(tb, aq, aj, au, s);
+    case MODE_BF16X3: return inv_rt_mode<MODE_BF16X3>(tb, aq, aj, au, s);
+    default: return -1;
+  }
+}
+
 int launch_forward(const Tables& tb, int mode, bool fused, const FwdArgs& a, void* scratch,
                    size_t sbytes, hipStream_t s) {
   switch (mode) {

@@ -16,6 +16,32 @@ __global__ __launch_bounds__(256) void k_inv(InvArgs a, const void* tabA0, const
                                     tabv, twp, lds);
 }
 
+// The three inverse transforms of a round trip in one launch: per member,
+// the sp2fc blocks (aq, table qi), the sp2fcdmu blocks (aj, table qj) and the
+// dv2uv blocks (au), each block the same work item as in its own kernel.
+// One launch instead of three removes two ramp-up/drain tails, and the
+// blocks of one member (which read the same spectral coefficients) run
+// close together.
+template <int T, int MODE, int V>
+__global__ __launch_bounds__(256) void k_inv_rt(InvArgs aq, InvArgs aj, InvArgs au, const void* tq,
+                                                const void* tj, const void* tv0, const void* tu,
+                                                long plane, const void* tabv, const void* twp,
+                                                int gq, int gj, int gu) {
+  __shared__ __align__(16) char lds[InvLds<T, MODE, true, V>::BYTES];
+  constexpr int NPB = TS<T>::NPAIR / InvCfg<T, MODE, V>::LP;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int per = NPB * (gq + gj + gu);
+  const int mem = b / per, r = b - mem * per;
+  if (r < NPB * gq)
+    inv_body<T, MODE, INV_SCALAR, true, V>(aq, mem * NPB * gq + r, tq, tu, plane, tabv, twp, lds);
+  else if (r < NPB * (gq + gj))
+    inv_body<T, MODE, INV_SCALAR, true, V>(aj, mem * NPB * gj + (r - NPB * gq), tj, tu, plane,
+                                           tabv, twp, lds);
+  else
+    inv_body<T, MODE, INV_UV, true, V>(au, mem * NPB * gu + (r - NPB * (gq + gj)), tv0, tu, plane,
+                                       tabv, twp, lds);
+}
+
 template <int T, int MODE, bool FUSED, int V>
 __global__ __launch_bounds__(256) void k_fwd(FwdArgs a, const void* tabv, const void* twp,
                                              void* scratch) {
@@ -78,6 +104,43 @@ static int inv_t(const Tables& tb, bool fused, const InvArgs& a, hipStream_t s) 
   return inv_tv<T, MODE, 0>(tb, fused, a, s);
 }
 
+template <int T, int MODE, int V>
+static int inv_rt_tv(const Tables& tb, const InvArgs& aq, const InvArgs& aj, const InvArgs& au,
+                     hipStream_t s) {
+  using C = InvCfg<T, MODE, V>;
+  constexpr int NPB = TS<T>::NPAIR / C::LP;
+  if (aq.nmem != aj.nmem || aq.nmem != au.nmem) return -1;
+  const int gq = (aq.nfield + C::FB - 1) / C::FB;
+  const int gj = (aj.nfield + C::FB - 1) / C::FB;
+  const int gu = (au.nlev + C::FB / 2 - 1) / (C::FB / 2);
+  const long nblk = (long)aq.nmem * NPB * (gq + gj + gu);
+  if (nblk <= 0) return 0;
+  const void *tq = nullptr, *tj = nullptr, *tv0 = nullptr, *tu = nullptr;
+  if (MODE == MODE_BF16X3) {
+    tq = tb.inv_bf[TQI];
+    tj = tb.inv_bf[TQJ];
+    tv0 = tb.inv_bf[TQV];
+    tu = tb.inv_bf[TQU];
+  } else if (MODE == MODE_F32MFMA) {
+    tq = tb.inv_f32[TQI];
+    tj = tb.inv_f32[TQJ];
+    tv0 = tb.inv_f32[TQV];
+    tu = tb.inv_f32[TQU];
+  }
+  const void* tv = (MODE == MODE_F64) ? (const void*)tb.filt64 : (const void*)tb.filt32;
+  const void* tw = (MODE == MODE_F64) ? (const void*)tb.tw64 : (const void*)tb.tw32;
+  k_inv_rt<T, MODE, V><<<(unsigned)nblk, 256, 0, s>>>(aq, aj, au, tq, tj, tv0, tu, tb.inv_elems,
+                                                       tv, tw, gq, gj, gu);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+template <int T, int MODE>
+static int inv_rt_t(const Tables& tb, const InvArgs& aq, const InvArgs& aj, const InvArgs& au,
+                    hipStream_t s) {
+  if (aq.var == 1) return inv_rt_tv<T, MODE, 1>(tb, aq, aj, au, s);
+  return inv_rt_tv<T, MODE, 0>(tb, aq, aj, au, s);
+}
+
 template <int T, int MODE>
 static size_t fwd_scratch_t(int nmem, int nfield) {
   using C = FwdCfg<T, MODE>;
@@ -128,6 +191,11 @@ static int fwd_t(const Tables& tb, bool fused, const FwdArgs& a, void* scratch, 
 template <int MODE_X>
 int inv_mode(const Tables& tb, bool fused, const InvArgs& a, hipStream_t s) {
   EXO_DISPATCH_T(inv_t, tb, fused, a, s)
+}
+template <int MODE_X>
+int inv_rt_mode(const Tables& tb, const InvArgs& aq, const InvArgs& aj, const InvArgs& au,
+                hipStream_t s) {
+  EXO_DISPATCH_T(inv_rt_t, tb, aq, aj, au, s)
 }
 template <int MODE_X>
 int fwd_mode(const Tables& tb, bool fused, const FwdArgs& a, void* scr, size_t sb,

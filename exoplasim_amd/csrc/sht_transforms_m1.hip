@@ -3,6 +3,8 @@
 
 namespace exo {
 template int inv_mode<MODE_F32>(const Tables&, bool, const InvArgs&, hipStream_t);
+template int inv_rt_mode<MODE_F32>(const Tables&, const InvArgs&, const InvArgs&, const InvArgs&,
+                                  hipStream_t);
 template int fwd_mode<MODE_F32>(const Tables&, bool, const FwdArgs&, void*, size_t, hipStream_t);
 template size_t scr_mode<MODE_F32>(const Tables&, int, int);
 }  // namespace exo

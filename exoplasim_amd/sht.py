@@ -295,6 +295,11 @@ class Plan:
                                       _ptr(sp_in), _ptr(gp), _ptr(sp_out), _stream()),
               "roundtrip")
 
+    def inverse_all(self, mode, sp_in, gp, nlev):
+        """The inverse half of a round trip in one launch (exo_sht_inverse_all)."""
+        check(lib().exo_sht_inverse_all(self._h, mode_id(mode), sp_in.shape[0], nlev,
+                                        _ptr(sp_in), _ptr(gp), _stream()), "inverse_all")
+
     def set_tiles(self, mode, inv_var=0, uv_var=0, fwd_var=0):
         check(lib().exo_sht_set_tiles(self._h, mode_id(mode), inv_var, uv_var, fwd_var),
               "set_tiles")

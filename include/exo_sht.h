@@ -154,12 +154,22 @@ int exo_sht_direct(exo_sht_plan_t plan, int mode, int nmem, const void *gp, long
  * fields. */
 int exo_sht_roundtrip(exo_sht_plan_t plan, int mode, int nmem, int nlev, const void *sp_in,
                       void *gp, void *sp_out, void *stream);
+/* The inverse half of exo_sht_roundtrip: the grid-point fields of the
+ * dynamics step (invlegd and invlega, src/legmod.f90:635-677) -- sp2fc of the
+ * nfl = 3*nlev+1 field-levels, sp2fcdmu of ln ps and dv2uv of the nlev
+ * levels, each fused with fc2gp, in one kernel launch; same layouts. */
+int exo_sht_inverse_all(exo_sht_plan_t plan, int mode, int nmem, int nlev, const void *sp_in,
+                        void *gp, void *stream);
 
-/* Tile variants of the three fused kernels (inverse sp2fc/sp2fcdmu,
- * inverse dv2uv, direct): 0 = default tiles, 1 = half the field-levels per
- * block (half the LDS, twice the blocks in flight).  exo_sht_autotune times
- * both on a round trip of nmem members x nlev levels, keeps the fastest of
- * each kernel for this mode and returns them in vars[3] (may be NULL). */
+/* Tile variants of the fused kernels: vars[0] the inverse sp2fc/sp2fcdmu
+ * kernel and the one-launch inverse of exo_sht_inverse_all /
+ * exo_sht_roundtrip, vars[1] the dv2uv kernel of exo_sht_inverse_uv,
+ * vars[2] the direct kernel; 0 = default tiles, 1 = half the field-levels
+ * per block (half the LDS, twice the blocks in flight).  exo_sht_autotune
+ * times both on a round trip of nmem members x nlev levels, keeps the
+ * fastest of each for this mode and returns them in vars[3] (may be NULL).
+ * Calls on one plan share its scratch buffers: calls that overlap on
+ * different streams race. */
 int exo_sht_set_tiles(exo_sht_plan_t plan, int mode, int inv_var, int uv_var, int fwd_var);
 int exo_sht_get_tiles(exo_sht_plan_t plan, int mode, int *vars);
 int exo_sht_autotune(exo_sht_plan_t plan, int mode, int nmem, int nlev, void *stream, int *vars);

@@ -133,6 +133,31 @@ def test_roundtrip_tile_variant(oracle, gpu, mode, ntru, nlev):
         p.close()
 
 
+@pytest.mark.parametrize("mode", ["f64", "bf16x3", "f32mfma"])
+@pytest.mark.parametrize("ntru,nlev", [(21, 10), (42, 20), (127, 4)])
+def test_inverse_all(plans, oracle, gpu, mode, ntru, nlev):
+    # the one-launch inverse half of the round trip, both tile variants
+    import torch
+
+    print(f"inverse_all T{ntru} L{nlev} {mode}", flush=True)
+    p = get_plan(plans, ntru)
+    nmem = 3
+    sp = ensemble(ntru, nlev, nmem, cfg=5)
+    g_ref = np.stack([oracle.roundtrip(ntru, nlev, sp[k])[0] for k in range(nmem)])
+    sp_in, gp, _ = p.alloc_roundtrip(nmem, nlev, mode)
+    sp_in.copy_(torch.from_numpy(sp).to(sp_in.dtype))
+    for var in (0, 1):
+        p.set_tiles(mode, var, 0, 0)
+        gp.fill_(float("nan"))
+        p.inverse_all(mode, sp_in, gp, nlev)
+        torch.cuda.synchronize()
+        g = gp.double().cpu().numpy()
+        for k in range(nmem):
+            e = rel_l2(g[k], g_ref[k])
+            assert e < TOL[mode], (var, k, e)
+    p.set_tiles(mode, 0, 0, 0)
+
+
 def test_autotune(oracle, gpu):
     print("autotune T42 L20 bf16x3", flush=True)
     p = sht.Plan(42)
