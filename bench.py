@@ -27,6 +27,13 @@ sys.path.insert(0, ROOT)
 NTRU, NLEV = 42, 20
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the two halves at this workload (128 members,
+# split-BF16), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this
+# script, each scaled by the counter's reading for a 256 MiB device copy
+# (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); the runs and the command are in
+# profiles/t42_l20_hbm_traffic.md.  Keyed by (mode, call, members).
+MEASURED_TRAFFIC = {}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -356,7 +363,8 @@ def main():
     dom_name, dom = max(head["kernels"].items(), key=lambda kv: kv[1]["ms"])
     achieved = dom["GBs"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": MEASURED_TRAFFIC.get((args.mode, dom_name, nmem)),
                 "kernel": dom_name, "kernel_ms": round(dom["ms"], 4),
                 "algorithmic_bytes_per_launch": dom["bytes"]}
     cpu = None
