@@ -242,9 +242,38 @@ def bench_emulator(args, ora):
     rms = float(torch.sqrt(torch.mean((h_em - h_ref) ** 2)))
     sig = float(torch.sqrt(torch.mean(h_ref ** 2)))
     rms_olr = float(torch.sqrt(torch.mean((o_em - o_ref) ** 2)))
-    t_col = time_loop(lambda: col(test), 20, 3) / 20
+    # steps captured in HIP graphs (10 steps per graph), so that the launch
+    # cost of the Python caller does not count; every replay is checked
+    # against the eager result
     outb = (torch.empty_like(h_em), torch.empty_like(o_em))
-    t_em = time_loop(lambda: em.predict(test, out=outb), 200, 10) / 200
+    em.predict(test, out=outb)
+    ref_em = (outb[0].clone(), outb[1].clone())
+    outc = col(test)
+    ref_col = (outc[0].clone(), outc[1].clone())
+
+    def graphed(fn, reps=10):
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            fn()  # warm-up on the capture stream
+        torch.cuda.current_stream().wait_stream(s)
+        with torch.cuda.graph(g):
+            for _ in range(reps):
+                res = fn()
+        return g, res
+
+    g_em, _ = graphed(lambda: em.predict(test, out=outb))
+    g_col, out_col = graphed(lambda: col(test))
+    for _ in range(3):
+        g_em.replay()
+        g_col.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(outb[0], ref_em[0]) and torch.equal(outb[1], ref_em[1])
+        assert torch.equal(out_col[0], ref_col[0]) and torch.equal(out_col[1], ref_col[1])
+    t_col = time_loop(g_col.replay, 5, 1) / 50
+    t_em = time_loop(g_em.replay, 20, 2) / 200
+    t_em_eager = time_loop(lambda: em.predict(test, out=outb), 200, 10) / 200
     # CPU: the oracle column (C restatement of lwr) on host threads
     import oracle as O
 
@@ -271,6 +300,8 @@ def bench_emulator(args, ora):
             "olr_rms_W_m2": round(rms_olr, 3),
             "columns_per_step": ncol,
             "emulator_columns_per_s": round(ncol / t_em, 1),
+            "emulator_columns_per_s_eager_python": round(ncol / t_em_eager, 1),
+            "timing": "10 steps per captured HIP graph, replays checked against eager results",
             "gpu_plain_column_columns_per_s": round(ncol / t_col, 1),
             "cpu_plain_column_columns_per_s": round(cpu_cols, 1), "cpu_cores": cores,
             "gain_vs_gpu_plain_column": round(t_col / t_em, 2),

@@ -427,8 +427,10 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
         if (d < 0) return;
         const int lat = pb * LP + pl;
         R* o = outm + (long)d * NLAT * N;
-        o[(long)lat * N + k] = v.x;
-        o[(long)(NLAT - 1 - lat) * N + k] = v.y;
+        // grid rows stream out: non-temporal, so that they do not evict the
+        // spectral operands and tables that the other blocks reuse from L2
+        __builtin_nontemporal_store(v.x, o + (long)lat * N + k);
+        __builtin_nontemporal_store(v.y, o + (long)(NLAT - 1 - lat) * N + k);
       };
       fft_run<N, +1, R>(fbuf, tws, nf, load, store);
     }

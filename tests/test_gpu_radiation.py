@@ -115,6 +115,24 @@ def test_emulator_fused_inference(gpu):
     e_o = np.linalg.norm(o1.double().cpu().numpy() - y[:, 20]) / np.linalg.norm(y[:, 20])
     print(f"fused vs oracle: heating {e_h:.2e}, olr {e_o:.2e}", flush=True)
     assert e_h < 2e-3 and e_o < 2e-3
+    # captured in a HIP graph: every replay, with eager work in between,
+    # gives the eager result
+    out = (torch.empty_like(h1), torch.empty_like(o1))
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        em.predict(cols, out=out)
+    torch.cuda.current_stream().wait_stream(s)
+    with torch.cuda.graph(g):
+        em.predict(cols, out=out)
+    for _ in range(3):
+        out[0].fill_(float("nan"))
+        out[1].fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out[0], h1) and torch.equal(out[1], o1)
 
 
 def test_emulator_training_and_gain(gpu):
