@@ -17,13 +17,16 @@ using RealT = typename std::conditional<MODE == MODE_F64, double, float>::type;
 // tile shapes
 // ---------------------------------------------------------------------------
 // V selects the tile variant the autotuner times: V = 1 halves the fields
-// per block (half the LDS, twice the blocks in flight).
+// per block (half the LDS, twice the blocks in flight); V = 2 keeps the
+// fields and runs the block with 512 threads (8 waves: twice the waves in
+// flight per CU for the same LDS, FFT batches of twice the rows).
 template <int T, int MODE, int V = 0>
 struct InvCfg {
   static constexpr int LP = 8;  // latitude pairs per tile (= 16 MFMA rows)
   static constexpr int FB0 =
       (T == 127) ? (MODE == MODE_F64 ? 2 : 4) : (MODE == MODE_F64 ? 4 : 8);
-  static constexpr int FB = (FB0 >> V) < 2 ? 2 : (FB0 >> V);
+  static constexpr int FB = (V == 1) ? ((FB0 >> 1) < 2 ? 2 : (FB0 >> 1)) : FB0;
+  static constexpr int NT = (V == 2) ? 512 : 256;  // threads per block
 };
 
 template <int T, int MODE, int V = 0>
@@ -33,8 +36,9 @@ struct FwdCfg {
                           : (T == 63) ? (MODE == MODE_F64 ? 24 : 48)
                                       : (MODE == MODE_F64 ? 16 : 32);
   static constexpr int FBF0 = (T == 21) ? 4 : (T == 42) ? (MODE == MODE_F64 ? 1 : 2) : 1;
-  static constexpr int FBF = (FBF0 >> V) < 1 ? 1 : (FBF0 >> V);
+  static constexpr int FBF = (V == 1) ? ((FBF0 >> 1) < 1 ? 1 : (FBF0 >> 1)) : FBF0;
   static constexpr int NPB = TS<T>::NPAIR / PB;
+  static constexpr int NT = (V == 2) ? 512 : 256;
 };
 
 template <typename R>
@@ -57,7 +61,7 @@ struct InvLds {
   using C = InvCfg<T, MODE, V>;
   static constexpr size_t FCL = sizeof(NS2<R>) * C::LP * C::FB * TS<T>::NM;
   static constexpr size_t FBUF =
-      FUSED ? sizeof(C2<R>) * fft_buf_elems(FftBatch<TS<T>::NLON>::NF * TS<T>::NLON) : 0;
+      FUSED ? sizeof(C2<R>) * fft_buf_elems(FftBatch<TS<T>::NLON, C::NT>::NF * TS<T>::NLON) : 0;
   static constexpr size_t TWS = FUSED ? sizeof(C2<R>) * TS<T>::NLON : 0;
   static constexpr size_t BYTES = FCL + FBUF + TWS;
 };
@@ -81,7 +85,8 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
   using R = RealT<MODE>;
   constexpr int NM = S::NM, N = S::NLON, NLAT = S::NLAT, NPB = S::NPAIR / 8;
   constexpr int LP = InvCfg<T, MODE, V>::LP, FB = InvCfg<T, MODE, V>::FB;
-  constexpr int NF = FftBatch<N>::NF;
+  constexpr int NT = InvCfg<T, MODE, V>::NT, NW = NT / 64;  // threads, waves
+  constexpr int NF = FftBatch<N, NT>::NF;
   constexpr int NTERM = (KIND == INV_UV) ? 2 : 1;
 
   using L = InvLds<T, MODE, FUSED, V>;
@@ -100,7 +105,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
 
   if (FUSED) {
     const C2<R>* tw = (const C2<R>*)twp;
-    for (int k = tid; k < N; k += 256) tws[k] = tw[k];
+    for (int k = tid; k < N; k += NT) tws[k] = tw[k];
   }
 
   // value of MFMA/GEMM column c (0..2FB-1) for spectral index w, term t
@@ -260,7 +265,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     };
     auto next_item = [&](Item it) -> Item {
       const int kbn = (NM - it.m + 31) / 32;
-      return (it.kb + 1 < kbn) ? Item{it.m, it.kb + 1} : Item{it.m + 4, 0};
+      return (it.kb + 1 < kbn) ? Item{it.m, it.kb + 1} : Item{it.m + NW, 0};
     };
     const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
     f32x4 acc1 = zero4, acc1b = zero4, acc2 = zero4, acc2b = zero4;
@@ -340,7 +345,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     constexpr int NPC = LP * 2 * FB;
     const int t0 = (KIND == INV_UV) ? TQV : a.table;
     const int t1 = TQU;
-    for (int idx = tid; idx < NPC * NM; idx += 256) {
+    for (int idx = tid; idx < NPC * NM; idx += NT) {
       const int pc = idx % NPC, m = idx / NPC;
       const int pl = pc % LP, c = pc / LP;
       const int pair = pb * LP + pl;
@@ -386,7 +391,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
 
   if constexpr (!FUSED) {
     // Fourier output fc(2, NLON/2, NLAT), zero above the truncation
-    for (int idx = tid; idx < LP * FB * (N / 2); idx += 256) {
+    for (int idx = tid; idx < LP * FB * (N / 2); idx += NT) {
       const int m = idx % (N / 2), pf = idx / (N / 2);
       const int fl = pf % FB, pl = pf / FB;
       const int d = dst_of(fl);
@@ -432,7 +437,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
         __builtin_nontemporal_store(v.x, o + (long)lat * N + k);
         __builtin_nontemporal_store(v.y, o + (long)(NLAT - 1 - lat) * N + k);
       };
-      fft_run<N, +1, R>(fbuf, tws, nf, load, store);
+      fft_run<N, +1, R, NT>(fbuf, tws, nf, load, store);
     }
   }
 }
@@ -455,7 +460,7 @@ struct FwdLds {
   static constexpr int RS = (C::PB + 1) * 2 * C::FBF;
   static constexpr size_t F = sizeof(R) * TS<T>::NM * 2 * RS;
   static constexpr size_t FBUF =
-      FUSED ? sizeof(C2<R>) * fft_buf_elems(FftBatch<TS<T>::NLON>::NF * TS<T>::NLON) : 0;
+      FUSED ? sizeof(C2<R>) * fft_buf_elems(FftBatch<TS<T>::NLON, C::NT>::NF * TS<T>::NLON) : 0;
   static constexpr size_t TWS = FUSED ? sizeof(C2<R>) * TS<T>::NLON : 0;
   static constexpr size_t BYTES = F + FBUF + TWS;
 };
@@ -475,7 +480,8 @@ __device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* 
   constexpr int NM = S::NM, N = S::NLON, NLAT = S::NLAT;
   constexpr int PB = C::PB, FBF = C::FBF, NPB = C::NPB;
   constexpr int NCOL = 2 * FBF;
-  constexpr int NF = FftBatch<N>::NF;
+  constexpr int NT = C::NT;
+  constexpr int NF = FftBatch<N, NT>::NF;
 
   // F[m][cls][pair][col]: cls 0 = fc_N + fc_S (E), cls 1 = fc_N - fc_S (O)
   using L = FwdLds<T, MODE, FUSED, V>;
@@ -493,7 +499,7 @@ __device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* 
 
   if (FUSED) {
     const C2<R>* tw = (const C2<R>*)twp;
-    for (int k = tid; k < N; k += 256) tws[k] = C2<R>{tw[k].x, -tw[k].y};
+    for (int k = tid; k < N; k += NT) tws[k] = C2<R>{tw[k].x, -tw[k].y};
   }
   __syncthreads();
 
@@ -511,10 +517,10 @@ __device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* 
         return C2<R>{src[(long)lat * N + k], src[(long)(NLAT - 1 - lat) * N + k]};
       };
       auto store = [&](int f, int k, C2<R> v) { fbuf[bix(f * N + k)] = v; };
-      fft_run<N, -1, R>(fbuf, tws, nf, load, store);
+      fft_run<N, -1, R, NT>(fbuf, tws, nf, load, store);
       // separate the two real rows, scale by 1/N, fold into E/O
       const R sc = R(0.5) / R(N);
-      for (int idx = tid; idx < nf * NM; idx += 256) {
+      for (int idx = tid; idx < nf * NM; idx += NT) {
         const int f = idx % nf, k = idx / nf;
         const C2<R> zk = fbuf[bix(f * N + k)];
         const C2<R> zn = fbuf[bix(f * N + ((N - k) % N))];
@@ -529,7 +535,7 @@ __device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* 
       __syncthreads();
     }
   } else {
-    for (int idx = tid; idx < PB * FBF * NM; idx += 256) {
+    for (int idx = tid; idx < PB * FBF * NM; idx += NT) {
       const int pf = idx % (PB * FBF), k = idx / (PB * FBF);
       const int fl = pf % FBF, pl = pf / FBF;
       const int field = g * FBF + fl;
@@ -554,7 +560,7 @@ __device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* 
     outp = (R*)scratch + ((long)pb * a.nmem + mem) * a.nfield * S::NRSP;
   const bool qtab = table_is_q(a.table);
   const R* tv = (const R*)tabv + (long)a.table * S::NPAIR * S::NCSP + (long)pb * PB * S::NCSP;
-  for (int w = tid; w < S::NCSP; w += 256) {
+  for (int w = tid; w < S::NCSP; w += NT) {
     const int m = m_of_w(T, w);
     const int n = m + (w - w0_of(T, m));
     const int cls = ((((n - m) & 1) == 0) != qtab) ? 0 : 1;

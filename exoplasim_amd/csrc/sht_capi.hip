@@ -366,8 +366,8 @@ extern "C" int exo_sht_roundtrip(exo_sht_plan_t p, int mode, int nmem, int nlev,
 
 extern "C" int exo_sht_set_tiles(exo_sht_plan_t p, int mode, int inv_var, int uv_var,
                                  int fwd_var) {
-  if (!p || !mode_ok(mode) || inv_var < 0 || inv_var > 1 || uv_var < 0 || uv_var > 1 ||
-      fwd_var < 0 || fwd_var > 1)
+  if (!p || !mode_ok(mode) || inv_var < 0 || inv_var > 2 || uv_var < 0 || uv_var > 2 ||
+      fwd_var < 0 || fwd_var > 2)
     return EXO_ERR_ARG;
   p->inv_var[mode] = inv_var;
   p->uv_var[mode] = uv_var;
@@ -452,7 +452,7 @@ extern "C" int exo_sht_autotune(exo_sht_plan_t p, int mode, int nmem, int nlev, 
   int rc = EXO_OK;
   float best[3] = {1e30f, 1e30f, 1e30f};
   int pick[3] = {0, 0, 0};
-  for (int v = 0; v < 2 && rc == EXO_OK; ++v) {
+  for (int v = 0; v < 3 && rc == EXO_OK; ++v) {
     p->inv_var[mode] = p->uv_var[mode] = p->fwd_var[mode] = v;
     for (int k = 0; k < 3 && rc == EXO_OK; ++k) {
       float ms_min = 1e30f;

@@ -208,10 +208,11 @@ template <> struct FftPlan<128> { static constexpr int NP = 2; static constexpr 
 template <> struct FftPlan<192> { static constexpr int NP = 3; static constexpr int R[4] = {8, 8, 3, 1}; };
 template <> struct FftPlan<384> { static constexpr int NP = 3; static constexpr int R[4] = {16, 8, 3, 1}; };
 
-// FFTs in flight per pass within a 256-thread block (about 2048 points)
-template <int N>
+// FFTs in flight per pass within an NT-thread block (about 8 points per
+// thread)
+template <int N, int NT = 256>
 struct FftBatch {
-  static constexpr int NF = (N == 64) ? 32 : (N == 128) ? 16 : (N == 192) ? 10 : 5;
+  static constexpr int NF = ((N == 64) ? 32 : (N == 128) ? 16 : (N == 192) ? 10 : 5) * (NT / 256);
 };
 
 // FFT work rows in LDS hold one pad element after every 32 complex values:
@@ -236,18 +237,18 @@ __device__ __forceinline__ int item_tid() {
   return t;
 }
 
-template <int N, int R, int NS, int S, bool FIRST, bool LAST, typename F,
+template <int N, int R, int NS, int S, bool FIRST, bool LAST, typename F, int NT,
           typename Load, typename Store>
 __device__ __forceinline__ void fft_pass(C2<F>* buf, const C2<F>* tw, int nf,
                                          Load& load, Store& store) {
   constexpr int NB = N / R;  // butterflies per FFT
-  constexpr int BPT = (FftBatch<N>::NF * NB + 255) / 256;
+  constexpr int BPT = (FftBatch<N, NT>::NF * NB + NT - 1) / NT;
   const int tid = item_tid();
   const int total = nf * NB;
   C2<F> v[BPT][R];
 #pragma unroll
   for (int t = 0; t < BPT; ++t) {
-    const int b = tid + t * 256;
+    const int b = tid + t * NT;
     if (b < total) {
       const int f = b / NB, j = b - f * NB;
 #pragma unroll
@@ -262,7 +263,7 @@ __device__ __forceinline__ void fft_pass(C2<F>* buf, const C2<F>* tw, int nf,
   if (!FIRST) __syncthreads();
 #pragma unroll
   for (int t = 0; t < BPT; ++t) {
-    const int b = tid + t * 256;
+    const int b = tid + t * NT;
     if (b < total) {
       const int f = b / NB, j = b - f * NB;
       const int jm = j % NS;
@@ -284,24 +285,25 @@ __device__ __forceinline__ void fft_pass(C2<F>* buf, const C2<F>* tw, int nf,
   __syncthreads();
 }
 
-// Full FFT of nf (<= FftBatch<N>::NF) sequences through the radix plan.
-template <int N, int S, typename F, typename Load, typename Store>
+// Full FFT of nf (<= FftBatch<N, NT>::NF) sequences through the radix plan
+// by a block of NT threads.
+template <int N, int S, typename F, int NT = 256, typename Load, typename Store>
 __device__ __forceinline__ void fft_run(C2<F>* buf, const C2<F>* tw, int nf, Load& load,
                                         Store& store) {
   using P = FftPlan<N>;
   constexpr int R0 = P::R[0], R1 = P::R[1], R2 = P::R[2], R3 = P::R[3];
   if constexpr (P::NP == 2) {
-    fft_pass<N, R0, 1, S, true, false, F>(buf, tw, nf, load, store);
-    fft_pass<N, R1, R0, S, false, true, F>(buf, tw, nf, load, store);
+    fft_pass<N, R0, 1, S, true, false, F, NT>(buf, tw, nf, load, store);
+    fft_pass<N, R1, R0, S, false, true, F, NT>(buf, tw, nf, load, store);
   } else if constexpr (P::NP == 3) {
-    fft_pass<N, R0, 1, S, true, false, F>(buf, tw, nf, load, store);
-    fft_pass<N, R1, R0, S, false, false, F>(buf, tw, nf, load, store);
-    fft_pass<N, R2, R0 * R1, S, false, true, F>(buf, tw, nf, load, store);
+    fft_pass<N, R0, 1, S, true, false, F, NT>(buf, tw, nf, load, store);
+    fft_pass<N, R1, R0, S, false, false, F, NT>(buf, tw, nf, load, store);
+    fft_pass<N, R2, R0 * R1, S, false, true, F, NT>(buf, tw, nf, load, store);
   } else {
-    fft_pass<N, R0, 1, S, true, false, F>(buf, tw, nf, load, store);
-    fft_pass<N, R1, R0, S, false, false, F>(buf, tw, nf, load, store);
-    fft_pass<N, R2, R0 * R1, S, false, false, F>(buf, tw, nf, load, store);
-    fft_pass<N, R3, R0 * R1 * R2, S, false, true, F>(buf, tw, nf, load, store);
+    fft_pass<N, R0, 1, S, true, false, F, NT>(buf, tw, nf, load, store);
+    fft_pass<N, R1, R0, S, false, false, F, NT>(buf, tw, nf, load, store);
+    fft_pass<N, R2, R0 * R1, S, false, false, F, NT>(buf, tw, nf, load, store);
+    fft_pass<N, R3, R0 * R1 * R2, S, false, true, F, NT>(buf, tw, nf, load, store);
   }
 }
 

@@ -9,7 +9,7 @@
 namespace exo {
 
 template <int T, int MODE, int KIND, bool FUSED, int V>
-__global__ __launch_bounds__(256) void k_inv(InvArgs a, const void* tabA0, const void* tabA1,
+__global__ __launch_bounds__((InvCfg<T, MODE, V>::NT)) void k_inv(InvArgs a, const void* tabA0, const void* tabA1,
                                              long plane, const void* tabv, const void* twp) {
   __shared__ __align__(16) char lds[InvLds<T, MODE, FUSED, V>::BYTES];
   inv_body<T, MODE, KIND, FUSED, V>(a, xcd_block(blockIdx.x, gridDim.x), tabA0, tabA1, plane,
@@ -23,7 +23,7 @@ __global__ __launch_bounds__(256) void k_inv(InvArgs a, const void* tabA0, const
 // blocks of one member (which read the same spectral coefficients) run
 // close together.
 template <int T, int MODE, int V>
-__global__ __launch_bounds__(256) void k_inv_rt(InvArgs aq, InvArgs aj, InvArgs au, const void* tq,
+__global__ __launch_bounds__((InvCfg<T, MODE, V>::NT)) void k_inv_rt(InvArgs aq, InvArgs aj, InvArgs au, const void* tq,
                                                 const void* tj, const void* tv0, const void* tu,
                                                 long plane, const void* tabv, const void* twp,
                                                 int gq, int gj, int gu) {
@@ -43,7 +43,7 @@ __global__ __launch_bounds__(256) void k_inv_rt(InvArgs aq, InvArgs aj, InvArgs 
 }
 
 template <int T, int MODE, bool FUSED, int V>
-__global__ __launch_bounds__(256) void k_fwd(FwdArgs a, const void* tabv, const void* twp,
+__global__ __launch_bounds__((FwdCfg<T, MODE, V>::NT)) void k_fwd(FwdArgs a, const void* tabv, const void* twp,
                                              void* scratch) {
   __shared__ __align__(16) char lds[FwdLds<T, MODE, FUSED, V>::BYTES];
   fwd_body<T, MODE, FUSED, V>(a, xcd_block(blockIdx.x, gridDim.x), tabv, twp, scratch, lds);
@@ -82,7 +82,7 @@ static int inv_tv(const Tables& tb, bool fused, const InvArgs& a, hipStream_t s)
   }
   const void* tv = (MODE == MODE_F64) ? (const void*)tb.filt64 : (const void*)tb.filt32;
   const void* tw = (MODE == MODE_F64) ? (const void*)tb.tw64 : (const void*)tb.tw32;
-  dim3 grid((unsigned)nblk), block(256);
+  dim3 grid((unsigned)nblk), block(InvCfg<T, MODE, V>::NT);
   if (a.kind == INV_UV) {
     if (fused)
       k_inv<T, MODE, INV_UV, true, V><<<grid, block, 0, s>>>(a, A0, A1, tb.inv_elems, tv, tw);
@@ -101,6 +101,7 @@ static int inv_tv(const Tables& tb, bool fused, const InvArgs& a, hipStream_t s)
 template <int T, int MODE>
 static int inv_t(const Tables& tb, bool fused, const InvArgs& a, hipStream_t s) {
   if (fused && a.var == 1) return inv_tv<T, MODE, 1>(tb, fused, a, s);
+  if (fused && a.var == 2) return inv_tv<T, MODE, 2>(tb, fused, a, s);
   return inv_tv<T, MODE, 0>(tb, fused, a, s);
 }
 
@@ -129,7 +130,7 @@ static int inv_rt_tv(const Tables& tb, const InvArgs& aq, const InvArgs& aj, con
   }
   const void* tv = (MODE == MODE_F64) ? (const void*)tb.filt64 : (const void*)tb.filt32;
   const void* tw = (MODE == MODE_F64) ? (const void*)tb.tw64 : (const void*)tb.tw32;
-  k_inv_rt<T, MODE, V><<<(unsigned)nblk, 256, 0, s>>>(aq, aj, au, tq, tj, tv0, tu, tb.inv_elems,
+  k_inv_rt<T, MODE, V><<<(unsigned)nblk, C::NT, 0, s>>>(aq, aj, au, tq, tj, tv0, tu, tb.inv_elems,
                                                        tv, tw, gq, gj, gu);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
@@ -138,6 +139,7 @@ template <int T, int MODE>
 static int inv_rt_t(const Tables& tb, const InvArgs& aq, const InvArgs& aj, const InvArgs& au,
                     hipStream_t s) {
   if (aq.var == 1) return inv_rt_tv<T, MODE, 1>(tb, aq, aj, au, s);
+  if (aq.var == 2) return inv_rt_tv<T, MODE, 2>(tb, aq, aj, au, s);
   return inv_rt_tv<T, MODE, 0>(tb, aq, aj, au, s);
 }
 
@@ -159,7 +161,7 @@ static int fwd_tv(const Tables& tb, bool fused, const FwdArgs& a, void* scratch,
     return -4;
   const void* tv = (MODE == MODE_F64) ? (const void*)tb.filt64 : (const void*)tb.filt32;
   const void* tw = (MODE == MODE_F64) ? (const void*)tb.tw64 : (const void*)tb.tw32;
-  dim3 grid((unsigned)nblk), block(256);
+  dim3 grid((unsigned)nblk), block(C::NT);
   if (fused)
     k_fwd<T, MODE, true, V><<<grid, block, 0, s>>>(a, tv, tw, scratch);
   else
@@ -176,6 +178,7 @@ template <int T, int MODE>
 static int fwd_t(const Tables& tb, bool fused, const FwdArgs& a, void* scratch, size_t sbytes,
                  hipStream_t s) {
   if (fused && a.var == 1) return fwd_tv<T, MODE, 1>(tb, fused, a, scratch, sbytes, s);
+  if (fused && a.var == 2) return fwd_tv<T, MODE, 2>(tb, fused, a, scratch, sbytes, s);
   return fwd_tv<T, MODE, 0>(tb, fused, a, scratch, sbytes, s);
 }
 

@@ -165,7 +165,8 @@ int exo_sht_inverse_all(exo_sht_plan_t plan, int mode, int nmem, int nlev, const
  * kernel and the one-launch inverse of exo_sht_inverse_all /
  * exo_sht_roundtrip, vars[1] the dv2uv kernel of exo_sht_inverse_uv,
  * vars[2] the direct kernel; 0 = default tiles, 1 = half the field-levels
- * per block (half the LDS, twice the blocks in flight).  exo_sht_autotune
+ * per block (half the LDS, twice the blocks in flight), 2 = default tiles
+ * on 512-thread blocks (twice the waves per block).  exo_sht_autotune
  * times both on a round trip of nmem members x nlev levels, keeps the
  * fastest of each for this mode and returns them in vars[3] (may be NULL).
  * Calls on one plan share its scratch buffers: calls that overlap on
@@ -204,7 +205,8 @@ int exo_sht_ens_run_groups(exo_sht_ens_t ens, void *stream);
 /* the same round trips one member at a time (4 launches per member) */
 int exo_sht_ens_run_members(exo_sht_ens_t ens, void *stream);
 /* tile variants per group, vars[2*k] inverse and vars[2*k+1] direct of
- * group k (0 or 1, as exo_sht_set_tiles); set_tiles captures the graph
+ * group k (0 or 1 as for exo_sht_set_tiles; the persistent kernels run
+ * 256-thread blocks); set_tiles captures the graph
  * again.  autotune times every combination per truncation, keeps the
  * fastest and returns it in vars (may be NULL). */
 int exo_sht_ens_get_tiles(exo_sht_ens_t ens, int *vars);

@@ -120,15 +120,34 @@ def test_roundtrip_vs_oracle(plans, oracle, gpu, mode, ntru, nlev):
 
 @pytest.mark.parametrize("mode", ["f64", "f32", "f32mfma", "bf16x3"])
 @pytest.mark.parametrize("ntru,nlev", [(21, 10), (42, 20), (127, 4)])
-def test_roundtrip_tile_variant(oracle, gpu, mode, ntru, nlev):
-    # the half-field tiles (variant 1) of all three fused kernels; nlev=20
-    # also leaves a partial last block for the odd field counts
-    print(f"roundtrip tiles=1 T{ntru} L{nlev} {mode}", flush=True)
+@pytest.mark.parametrize("var", [1, 2])
+def test_roundtrip_tile_variant(oracle, gpu, mode, ntru, nlev, var):
+    # variant 1: half the field-levels per block; variant 2: 512-thread
+    # blocks; all three fused kernels.  nlev=20 also leaves a partial last
+    # block for the odd field counts
+    print(f"roundtrip tiles={var} T{ntru} L{nlev} {mode}", flush=True)
     p = sht.Plan(ntru)
     try:
-        p.set_tiles(mode, 1, 1, 1)
-        assert p.tiles(mode) == (1, 1, 1)
+        p.set_tiles(mode, var, var, var)
+        assert p.tiles(mode) == (var, var, var)
         _check_roundtrip(p, oracle, mode, ntru, nlev, nmem=3)
+        # the single-kind inverse kernels (sp2fc, sp2fcdmu, dv2uv) with the
+        # same tiles against the grid fields of the oracle's round trip
+        import torch
+
+        sp = ensemble(ntru, nlev, 2, cfg=6)
+        g_ref, _ = _oracle_roundtrip(oracle, ntru, nlev, sp)
+        sp_in, gp, _ = p.alloc_roundtrip(2, nlev, mode)
+        sp_in.copy_(torch.from_numpy(sp).to(sp_in.dtype))
+        gp.fill_(float("nan"))
+        nfl = 3 * nlev + 1
+        p.inverse(mode, sp_in, gp, nfl, 0, 0, "qi")
+        p.inverse(mode, sp_in, gp, 1, nfl - 1, nfl, "qj")
+        p.inverse_uv(mode, sp_in, gp, nlev, 0, nlev, nfl + 1, nfl + 1 + nlev)
+        torch.cuda.synchronize()
+        g = gp.double().cpu().numpy()
+        for k in range(2):
+            assert rel_l2(g[k], g_ref[k]) < TOL[mode], (k, rel_l2(g[k], g_ref[k]))
     finally:
         p.close()
 
@@ -146,7 +165,7 @@ def test_inverse_all(plans, oracle, gpu, mode, ntru, nlev):
     g_ref = np.stack([oracle.roundtrip(ntru, nlev, sp[k])[0] for k in range(nmem)])
     sp_in, gp, _ = p.alloc_roundtrip(nmem, nlev, mode)
     sp_in.copy_(torch.from_numpy(sp).to(sp_in.dtype))
-    for var in (0, 1):
+    for var in (0, 1, 2):
         p.set_tiles(mode, var, 0, 0)
         gp.fill_(float("nan"))
         p.inverse_all(mode, sp_in, gp, nlev)
@@ -163,10 +182,10 @@ def test_autotune(oracle, gpu):
     p = sht.Plan(42)
     try:
         v = p.autotune("bf16x3", 16, 20)
-        assert all(x in (0, 1) for x in v) and p.tiles("bf16x3") == v
+        assert all(x in (0, 1, 2) for x in v) and p.tiles("bf16x3") == v
         _check_roundtrip(p, oracle, "bf16x3", 42, 20)
         with pytest.raises(sht.ExoError):
-            p.set_tiles("bf16x3", 2, 0, 0)
+            p.set_tiles("bf16x3", 3, 0, 0)
     finally:
         p.close()
 
