@@ -36,14 +36,30 @@ __device__ __forceinline__ uint16_t f2bf(float x) {  // round to nearest even
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-// GELU (erf form) and its derivative
+// GELU (erf form) and its derivative.  erf by Abramowitz & Stegun 7.1.26:
+// erf(x) = 1 - (a1 t + ... + a5 t^5) exp(-x^2), t = 1 / (1 + p x) for x >= 0,
+// absolute error <= 1.5e-7 -- below the bf16 rounding that follows every
+// activation by four orders of magnitude, and a third of the cost of the
+// correctly rounded erff.  The derivative shares exp(-z^2/2).
+__device__ __forceinline__ float erf_as(float x, float& e) {
+  const float ax = fabsf(x);
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, ax, 1.0f));
+  const float poly =
+      t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.061405429f, -1.453152027f), 1.421413741f),
+                       -0.284496736f),
+               0.254829592f);
+  e = __expf(-ax * ax);
+  const float r = 1.0f - poly * e;
+  return copysignf(r, x);
+}
 __device__ __forceinline__ float gelu(float z) {
-  return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+  float e;
+  return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f, e));
 }
 __device__ __forceinline__ float gelu_d(float z) {
-  const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
-  const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
-  return cdf + z * pdf;
+  float e;  // exp(-z^2 / 2)
+  const float cdf = 0.5f * (1.0f + erf_as(z * 0.70710678118654752f, e));
+  return cdf + z * 0.39894228040143268f * e;
 }
 
 __device__ __forceinline__ f32x4 mfma(const V8& a, const V8& b, f32x4 c) {
@@ -242,59 +258,86 @@ inline unsigned nb(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // ---------------------------------------------------------------------------
 // fused inference: features -> 4 layers -> heating rates and OLR, one block
-// per 32 columns, activations in LDS, weights streamed from L2
+// of 8 waves per 32 columns, activations in LDS, weights streamed from L2.
+// Each layer is computed transposed, C^T[n][m] = W[n][:] . H[m][:], so that a
+// lane's four accumulator values (consecutive n of one column m) go to LDS
+// as one 8-byte store of the row-major activations.
 // ---------------------------------------------------------------------------
 constexpr int EM_NLEV = 20, EM_KIN = 64, EM_NH = 256, EM_NOUT = 21, EM_NOUTP = 32;
 constexpr int EM_MT = 32;             // columns per block
+constexpr int EM_NT = 512;            // threads per block (8 waves)
 constexpr int EM_LDH = EM_NH + 8;     // padded LDS row (bf16) of a hidden layer
 constexpr int EM_LDX = EM_KIN + 8;
 
-// one hidden layer: Hout[32][256] = gelu(Hin[32][K] W^T + b); wave w owns
-// output columns [64w, 64w + 64)
-template <int K, int LDI>
-__device__ __forceinline__ void em_hidden(const uint16_t* hin, const uint16_t* W, const float* b,
-                                          uint16_t* hout) {
+// this wave's weight fragments of one layer (output features [32w, 32w+32),
+// K/32 x 2 x 16 bytes per lane), requested ahead of the layer that uses them
+template <int K>
+__device__ __forceinline__ void em_load(const uint16_t* W, V8 (&wa)[K / 32][2]) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int q = lane >> 4, c = lane & 15;
-  f32x4 acc[2][4];
+  const uint16_t* wr = W + (long)(wv * 32 + c) * K + q * 8;
+#pragma unroll
+  for (int ks = 0; ks < K / 32; ++ks)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) wa[ks][i].q = *(const uint4*)(wr + (long)i * 16 * K + ks * 32);
+}
+
+// one hidden layer: Hout[32][256] = gelu(Hin[32][K] W^T + b); wave w owns
+// output features [32w, 32w + 32)
+template <int K, int LDI>
+__device__ __forceinline__ void em_hidden(const uint16_t* hin, const V8 (&wa)[K / 32][2],
+                                          const float* b, uint16_t* hout) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int q = lane >> 4, c = lane & 15;
+  const int n0 = wv * 32;
+  f32x4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const uint16_t* wr = W + (long)(wv * 64 + c) * K + q * 8;
-#pragma unroll 2
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    V8 a[2], bb[4];
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int i = 0; i < 2; ++i) a[i].q = *(const uint4*)(hin + (i * 16 + c) * LDI + k0 + q * 8);
+  for (int ks = 0; ks < K / 32; ++ks) {
+    V8 h[2];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) bb[j].q = *(const uint4*)(wr + (long)j * 16 * K + k0);
+    for (int j = 0; j < 2; ++j)
+      h[j].q = *(const uint4*)(hin + (j * 16 + c) * LDI + ks * 32 + q * 8);
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = mfma(a[i], bb[j], acc[i][j]);
+      for (int j = 0; j < 2; ++j) acc[i][j] = mfma(wa[ks][i], h[j], acc[i][j]);
   }
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int n = wv * 64 + j * 16 + c;
-    const float bn = b[n];
+  for (int i = 0; i < 2; ++i) {
+    const int nb = n0 + i * 16 + 4 * q;  // this lane's 4 consecutive features
+    const float4 bn = *(const float4*)(b + nb);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) hout[(i * 16 + 4 * q + r) * EM_LDH + n] = f2bf(gelu(acc[i][j][r] + bn));
+    for (int j = 0; j < 2; ++j) {
+      const int m = j * 16 + c;
+      uint2 pk;
+      pk.x = (uint32_t)f2bf(gelu(acc[i][j][0] + bn.x)) |
+             ((uint32_t)f2bf(gelu(acc[i][j][1] + bn.y)) << 16);
+      pk.y = (uint32_t)f2bf(gelu(acc[i][j][2] + bn.z)) |
+             ((uint32_t)f2bf(gelu(acc[i][j][3] + bn.w)) << 16);
+      *(uint2*)(hout + m * EM_LDH + nb) = pk;
+    }
   }
 }
 
-__global__ __launch_bounds__(256) void k_emulate(exo_lw_net net, int ncol, const float* dt,
-                                                 const float* dq, const float* dp,
-                                                 const float* co2, float* heat, float* olr) {
+__global__ __launch_bounds__(EM_NT) void k_emulate(exo_lw_net net, int ncol, const float* dt,
+                                                   const float* dq, const float* dp,
+                                                   const float* co2, float* heat, float* olr) {
   __shared__ __align__(16) uint16_t sx[EM_MT * EM_LDX];
   __shared__ __align__(16) uint16_t sh0[EM_MT * EM_LDH];
   __shared__ __align__(16) uint16_t sh1[EM_MT * EM_LDH];
   const int tid = threadIdx.x;
   const int col0 = blockIdx.x * EM_MT;
+  // the weights of the first two layers are on their way while the features
+  // are computed; the third layer's while the second runs
+  V8 w1[EM_KIN / 32][2], w2[EM_NH / 32][2], w3[EM_NH / 32][2];
+  em_load<EM_KIN>((const uint16_t*)net.w[0], w1);
+  em_load<EM_NH>((const uint16_t*)net.w[1], w2);
   // features (exoplasim_amd.radiation.features), column index fastest
-  for (int idx = tid; idx < EM_MT * EM_KIN; idx += 256) {
+  for (int idx = tid; idx < EM_MT * EM_KIN; idx += EM_NT) {
     const int m = idx % EM_MT, f = idx / EM_MT;
     const int col = min(col0 + m, ncol - 1);
     float v = 0.f;
@@ -311,38 +354,40 @@ __global__ __launch_bounds__(256) void k_emulate(exo_lw_net net, int ncol, const
     sx[m * EM_LDX + f] = f2bf(v);
   }
   __syncthreads();
-  em_hidden<EM_KIN, EM_LDX>(sx, (const uint16_t*)net.w[0], net.b[0], sh0);
+  em_hidden<EM_KIN, EM_LDX>(sx, w1, net.b[0], sh0);
   __syncthreads();
-  em_hidden<EM_NH, EM_LDH>(sh0, (const uint16_t*)net.w[1], net.b[1], sh1);
+  em_load<EM_NH>((const uint16_t*)net.w[2], w3);
+  em_hidden<EM_NH, EM_LDH>(sh0, w2, net.b[1], sh1);
   __syncthreads();
-  em_hidden<EM_NH, EM_LDH>(sh1, (const uint16_t*)net.w[2], net.b[2], sh0);
+  em_hidden<EM_NH, EM_LDH>(sh1, w3, net.b[2], sh0);
   __syncthreads();
-  // output layer 32 x 32: wave w takes row tile w>>1, column tile w&1
+  // output layer, transposed: waves 0..3 take feature tile wv&1 (n) and
+  // column tile wv>>1 (m)
   const int lane = tid & 63, wv = tid >> 6;
+  if (wv >= 4) return;
   const int q = lane >> 4, c = lane & 15;
-  const int rt = wv >> 1, ct = wv & 1;
+  const int nt = wv & 1, mt = wv >> 1;
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  const uint16_t* W4 = (const uint16_t*)net.w[3] + (long)(ct * 16 + c) * EM_NH + q * 8;
-#pragma unroll 4
-  for (int k0 = 0; k0 < EM_NH; k0 += 32) {
-    V8 a, bb;
-    a.q = *(const uint4*)(sh0 + (rt * 16 + c) * EM_LDH + k0 + q * 8);
-    bb.q = *(const uint4*)(W4 + k0);
-    acc = mfma(a, bb, acc);
-  }
-  const int n = ct * 16 + c;
-  if (n < EM_NOUT) {
-    const float bn = net.b[3][n], sd = net.ystd[n], mu = net.ymean[n];
+  const uint16_t* W4 = (const uint16_t*)net.w[3] + (long)(nt * 16 + c) * EM_NH + q * 8;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int col = col0 + rt * 16 + 4 * q + r;
-      if (col >= ncol) continue;
-      const float y = (acc[r] + bn) * sd + mu;
-      if (n < EM_NLEV)
-        heat[(long)n * ncol + col] = y / (dp[col] / 1e5f);
-      else
-        olr[col] = y;
-    }
+  for (int k0 = 0; k0 < EM_NH; k0 += 32) {
+    V8 w, h;
+    w.q = *(const uint4*)(W4 + k0);
+    h.q = *(const uint4*)(sh0 + (mt * 16 + c) * EM_LDH + k0 + q * 8);
+    acc = mfma(w, h, acc);
+  }
+  const int col = col0 + mt * 16 + c;
+  if (col >= ncol) return;
+  const float ps = dp[col] / 1e5f;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int n = nt * 16 + 4 * q + r;
+    if (n >= EM_NOUT) continue;
+    const float y = (acc[r] + net.b[3][n]) * net.ystd[n] + net.ymean[n];
+    if (n < EM_NLEV)
+      heat[(long)n * ncol + col] = y / ps;
+    else
+      olr[col] = y;
   }
 }
 
@@ -406,8 +451,8 @@ extern "C" int exo_lw_emulate(const exo_lw_net* net, int ncol, const float* dt, 
     return EXO_RAD_ERR_ARG;
   for (int l = 0; l < 4; ++l)
     if (!net->w[l] || !net->b[l]) return EXO_RAD_ERR_ARG;
-  k_emulate<<<nb(ncol, EM_MT), 256, 0, (hipStream_t)stream>>>(*net, ncol, dt, dq, dp, dqco2,
-                                                               heat, olr);
+  k_emulate<<<nb(ncol, EM_MT), EM_NT, 0, (hipStream_t)stream>>>(*net, ncol, dt, dq, dp, dqco2,
+                                                                 heat, olr);
   return hipGetLastError() == hipSuccess ? EXO_RAD_OK : EXO_RAD_ERR_HIP;
 }
 
