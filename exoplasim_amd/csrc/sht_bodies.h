@@ -315,28 +315,46 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
         acc1 = acc1b = acc2 = acc2b = zero4;
       }
     };
-    // three operand buffers in rotation, unrolled so that no buffer is
-    // copied: two items are in flight while one is multiplied
-    Frag f0, f1, f2;
-    Item i0 = {wv, 0};
-    Item i1 = next_item(i0), i2 = next_item(i1);
-    if (i0.m <= T) fetch(i0.m, i0.kb, f0.b, f0.a0, f0.a1);
-    if (i1.m <= T) fetch(i1.m, i1.kb, f1.b, f1.a0, f1.a1);
-    while (i0.m <= T) {
-      step(i0, f0, f2, i2);
-      i0 = i1;
-      i1 = i2;
-      i2 = next_item(i2);
-      if (i0.m > T) break;
-      step(i0, f1, f0, i2);
-      i0 = i1;
-      i1 = i2;
-      i2 = next_item(i2);
-      if (i0.m > T) break;
-      step(i0, f2, f1, i2);
-      i0 = i1;
-      i1 = i2;
-      i2 = next_item(i2);
+    if constexpr (NTERM == 1) {
+      // three operand buffers in rotation, unrolled so that no buffer is
+      // copied: two items are in flight while one is multiplied
+      Frag f0, f1, f2;
+      Item i0 = {wv, 0};
+      Item i1 = next_item(i0), i2 = next_item(i1);
+      if (i0.m <= T) fetch(i0.m, i0.kb, f0.b, f0.a0, f0.a1);
+      if (i1.m <= T) fetch(i1.m, i1.kb, f1.b, f1.a0, f1.a1);
+      while (i0.m <= T) {
+        step(i0, f0, f2, i2);
+        i0 = i1;
+        i1 = i2;
+        i2 = next_item(i2);
+        if (i0.m > T) break;
+        step(i0, f1, f0, i2);
+        i0 = i1;
+        i1 = i2;
+        i2 = next_item(i2);
+        if (i0.m > T) break;
+        step(i0, f2, f1, i2);
+        i0 = i1;
+        i1 = i2;
+        i2 = next_item(i2);
+      }
+    } else {
+      // dv2uv carries two tables per item: two buffers (one item ahead)
+      // keep the registers within four waves per SIMD
+      Frag f0, f1;
+      Item i0 = {wv, 0};
+      Item i1 = next_item(i0);
+      if (i0.m <= T) fetch(i0.m, i0.kb, f0.b, f0.a0, f0.a1);
+      while (i0.m <= T) {
+        step(i0, f0, f1, i1);
+        i0 = i1;
+        i1 = next_item(i1);
+        if (i0.m > T) break;
+        step(i0, f1, f0, i1);
+        i0 = i1;
+        i1 = next_item(i1);
+      }
     }
   } else {
     // vector-ALU Legendre: one (pair, column, m) sum per thread
