@@ -578,6 +578,7 @@ __device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* 
     outp = (R*)scratch + ((long)pb * a.nmem + mem) * a.nfield * S::NRSP;
   const bool qtab = table_is_q(a.table);
   const R* tv = (const R*)tabv + (long)a.table * S::NPAIR * S::NCSP + (long)pb * PB * S::NCSP;
+  const __amdgpu_buffer_rsrc_t trs = buffer_rsrc(tv, PB * S::NCSP * (int)sizeof(R));
   for (int w = tid; w < S::NCSP; w += NT) {
     const int m = m_of_w(T, w);
     const int n = m + (w - w0_of(T, m));
@@ -588,7 +589,7 @@ __device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* 
     for (int c = 0; c < NCOL; ++c) acc[c] = R(0);
 #pragma unroll 4
     for (int pl = 0; pl < PB; ++pl) {
-      const R p = tv[(long)pl * S::NCSP + w];
+      const R p = buffer_load<R>(trs, w * (int)sizeof(R), pl * S::NCSP * (int)sizeof(R));
 #pragma unroll
       for (int c = 0; c < NCOL; ++c) acc[c] += p * fr[pl * NCOL + c];
     }

@@ -70,6 +70,21 @@ __host__ __device__ constexpr int pk_blkoff(int ntru, int m) {
   return pk_cum(ntru + 1) - pk_cum(ntru + 1 - m);
 }
 
+// Raw buffer loads: the per-lane byte offset in a VGPR and a wave-uniform
+// byte offset in an SGPR, so a loop over a uniform stride (a table's
+// latitude rows) costs no vector address arithmetic.  Out-of-range loads
+// return 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, 0x00020000);
+}
+template <typename R>
+__device__ __forceinline__ R buffer_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  if constexpr (sizeof(R) == 4)
+    return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  else
+    return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2: block b runs on XCD b % 8.  This renumbering gives each XCD one
 // contiguous range of logical blocks, so blocks with neighbouring logical
