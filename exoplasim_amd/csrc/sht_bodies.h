@@ -145,12 +145,14 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     // wave wv takes m = wv, wv+4, ...; each (m, 32-wide n block) is one
     // work item, and the operands of the next item are fetched before the
     // current one is multiplied.
-    const int lane = tid & 63, wv = tid >> 6;
+    // the wave index is read into a scalar register, so that the item loop
+    // (m, kb), its branches and the operand offsets are wave-uniform scalar
+    // work rather than per-lane vector arithmetic
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int q = lane >> 4, c = lane & 15;
     // column c of B: SCALAR field g*FB + c/2; UV level g*FB/2 + c/4 of
     // vorticity ((c/2)&1 == 0) or divergence; component c&1
     const R* bp = nullptr;  // F32MFMA: the column's fp32 coefficients
-    const uint16_t* pbp = nullptr;  // BF16X3: the column's pre-split operand
     bool isvor = false;
     int pf = -1;  // field of column c
     if (KIND == INV_SCALAR) {
@@ -164,12 +166,29 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
         isvor = (s == 0) && ((c & 1) == 0);
       }
     }
-    if (pf >= 0) {
-      if constexpr (MODE == MODE_BF16X3)
-        pbp = (const uint16_t*)a.bpk + mem * a.bpk_stride +
-              (long)(pf - a.bpk_f0) * (pk_blocks(T) * PK_BLK) + (c & 1) * 32 + q * 8;
-      else
-        bp = spm + (long)pf * S::NRSP + (c & 1);
+    if constexpr (MODE == MODE_F32MFMA) {
+      if (pf >= 0) bp = spm + (long)pf * S::NRSP + (c & 1);
+    }
+    // Operands through buffer loads: a per-lane byte offset (loop-invariant)
+    // plus a wave-uniform byte offset per item.
+    // A: the tables of pair block pb; item (m, kb) at ((m KB + kb) 512)
+    // elements, planes `plane` elements apart.
+    constexpr int AE = (MODE == MODE_BF16X3) ? 2 : 4;  // bytes per table element
+    const long abase = (long)pb * NM * S::KB * 512;
+    const int abytes = (int)(((MODE == MODE_BF16X3) ? 3 : 1) * plane - abase) * AE;
+    const __amdgpu_buffer_rsrc_t rsA0 = buffer_rsrc((const char*)tabA0 + abase * AE, abytes);
+    const __amdgpu_buffer_rsrc_t rsA1 =
+        buffer_rsrc((const char*)(NTERM == 2 ? tabA1 : tabA0) + abase * AE, abytes);
+    const int avoff = (MODE == MODE_BF16X3) ? (c * 32 + q * 8) * 2 : (q * 16 + c) * 32;
+    const int aplane = (int)plane * AE;
+    // B (BF16X3): the member's pre-split operand; columns without a field
+    // read past the buffer, which returns zeros
+    __amdgpu_buffer_rsrc_t rsB = rsA0;
+    int bvoff = BUF_NONE;
+    if constexpr (MODE == MODE_BF16X3) {
+      rsB = buffer_rsrc((const uint16_t*)a.bpk + mem * a.bpk_stride, (int)a.bpk_stride * 2);
+      if (pf >= 0)
+        bvoff = ((pf - a.bpk_f0) * (pk_blocks(T) * PK_BLK) + (c & 1) * 32 + q * 8) * 2;
     }
     const R pv = (R)a.plavor;
     // B fragment of one item: 8 fp32 values (F32MFMA, n = 4j + q) or the
@@ -180,21 +199,16 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     };
     auto fetch = [&](int m, int kb, BFrag& br, uint4* a0, uint4* a1) {
       if constexpr (MODE == MODE_BF16X3) {
-        if (pbp) {
-          const uint16_t* src = pbp + (pk_blkoff(T, m) + kb) * PK_BLK;
+        const int bso = ((pk_blkoff(T, m) + kb) * PK_BLK) * 2;
 #pragma unroll
-          for (int p = 0; p < 3; ++p) br.p[p].q = *(const uint4*)(src + p * 64);
-          if (isvor && m == 0 && kb == 0 && q == 0) {
-            // absolute -> relative vorticity at (m=0, n=1, re): the split is
-            // exact, so the sum of the planes is the fp32 coefficient
-            const float x = __uint_as_float((uint32_t)br.p[0].s[1] << 16) +
-                            __uint_as_float((uint32_t)br.p[1].s[1] << 16) +
-                            __uint_as_float((uint32_t)br.p[2].s[1] << 16);
-            split3(x - (float)pv, br.p[0].s[1], br.p[1].s[1], br.p[2].s[1]);
-          }
-        } else {
-#pragma unroll
-          for (int p = 0; p < 3; ++p) br.p[p].q = uint4{0u, 0u, 0u, 0u};
+        for (int p = 0; p < 3; ++p) br.p[p].q = buffer_load_x4(rsB, bvoff, bso + p * 128);
+        if (m == 0 && kb == 0 && isvor && q == 0) {
+          // absolute -> relative vorticity at (m=0, n=1, re): the split is
+          // exact, so the sum of the planes is the fp32 coefficient
+          const float x = __uint_as_float((uint32_t)br.p[0].s[1] << 16) +
+                          __uint_as_float((uint32_t)br.p[1].s[1] << 16) +
+                          __uint_as_float((uint32_t)br.p[2].s[1] << 16);
+          split3(x - (float)pv, br.p[0].s[1], br.p[1].s[1], br.p[2].s[1]);
         }
       } else {
         const int w0 = w0_of(T, m);
@@ -209,25 +223,21 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
           br.f[j] = (float)x;
         }
       }
-      const long blk = ((long)(pb * NM + m) * S::KB + kb) * 512;
+      const int aso = (m * S::KB + kb) * 512 * AE;
       if constexpr (MODE == MODE_BF16X3) {
-        const uint16_t* A0 = (const uint16_t*)tabA0 + blk + c * 32 + q * 8;
 #pragma unroll
-        for (int p = 0; p < 3; ++p) a0[p] = *(const uint4*)(A0 + p * plane);
+        for (int p = 0; p < 3; ++p) a0[p] = buffer_load_x4(rsA0, avoff, aso + p * aplane);
         if (NTERM == 2) {
-          const uint16_t* A1 = (const uint16_t*)tabA1 + blk + c * 32 + q * 8;
 #pragma unroll
-          for (int p = 0; p < 3; ++p) a1[p] = *(const uint4*)(A1 + p * plane);
+          for (int p = 0; p < 3; ++p) a1[p] = buffer_load_x4(rsA1, avoff, aso + p * aplane);
         }
       } else {
         // f32 operand stored as [q][row][8] so one lane reads 8 floats
-        const float* A0 = (const float*)tabA0 + blk + (q * 16 + c) * 8;
-        a0[0] = *(const uint4*)A0;
-        a0[1] = *(const uint4*)(A0 + 4);
+        a0[0] = buffer_load_x4(rsA0, avoff, aso);
+        a0[1] = buffer_load_x4(rsA0, avoff, aso + 16);
         if (NTERM == 2) {
-          const float* A1 = (const float*)tabA1 + blk + (q * 16 + c) * 8;
-          a1[0] = *(const uint4*)A1;
-          a1[1] = *(const uint4*)(A1 + 4);
+          a1[0] = buffer_load_x4(rsA1, avoff, aso);
+          a1[1] = buffer_load_x4(rsA1, avoff, aso + 16);
         }
       }
     };
@@ -288,13 +298,10 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
           f32x4 p2;
 #pragma unroll
           for (int i = 0; i < 4; ++i) p2[i] = __shfl(acc2[i], plane_, 64);
-          const int r4 = c & 3;
+          // column c & 3 = 0: acc1 + p2, 1: acc1 - p2, 2: p2 - acc1, 3: -p2 - acc1
+          const float sa = (c & 2) ? -1.f : 1.f, sb = (c & 1) ? -1.f : 1.f;
 #pragma unroll
-          for (int i = 0; i < 4; ++i)
-            out[i] = (r4 == 0) ? acc1[i] + p2[i]
-                   : (r4 == 1) ? acc1[i] - p2[i]
-                   : (r4 == 2) ? p2[i] - acc1[i]
-                               : -p2[i] - acc1[i];
+          for (int i = 0; i < 4; ++i) out[i] = sa * acc1[i] + sb * p2[i];
         }
         const int fl = c >> 1, ri = c & 1;
         if (fl < FB) {
@@ -302,14 +309,10 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
           for (int h = 0; h < 2; ++h) {
             const int pl = 2 * q + h;
             const float sy = out[2 * h], an = out[2 * h + 1];
-            NS2<R>& e = fcl[(pl * FB + fl) * NM + m];
-            if (ri == 0) {
-              e.n.x = sy + an;
-              e.s.x = sy - an;
-            } else {
-              e.n.y = sy + an;
-              e.s.y = sy - an;
-            }
+            // component ri of the northern and southern coefficient
+            R* e = (R*)&fcl[(pl * FB + fl) * NM + m];
+            e[ri] = sy + an;
+            e[2 + ri] = sy - an;
           }
         }
         acc1 = acc1b = acc2 = acc2b = zero4;

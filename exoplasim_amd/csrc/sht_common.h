@@ -84,6 +84,12 @@ __device__ __forceinline__ R buffer_load(__amdgpu_buffer_rsrc_t r, int voff, int
   else
     return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
+__device__ __forceinline__ uint4 buffer_load_x4(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  return uint4{v[0], v[1], v[2], v[3]};
+}
+// a per-lane byte offset that no buffer covers: such a load returns 0
+constexpr int BUF_NONE = 0x7ff00000;
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2: block b runs on XCD b % 8.  This renumbering gives each XCD one
