@@ -53,15 +53,17 @@ __device__ __forceinline__ f32x4 mfma_bf16(const BF8& a, const BF8& b, f32x4 c) 
 // ---------------------------------------------------------------------------
 // inverse transform
 // ---------------------------------------------------------------------------
-// LDS of one inverse block: Fourier coefficients of LP pairs x FB fields,
-// plus the FFT work rows and twiddles when fused
+// LDS of one inverse block.  Unfused: the Fourier coefficients of LP pairs
+// x FB fields (north and south).  Fused: the FFT rows of those LP x FB
+// (pair, field) rows, z_k = fc_N(k) + i fc_S(k) with its Hermitian extension,
+// which the Legendre sums write directly, plus the twiddles.
 template <int T, int MODE, bool FUSED, int V>
 struct InvLds {
   using R = RealT<MODE>;
   using C = InvCfg<T, MODE, V>;
-  static constexpr size_t FCL = sizeof(NS2<R>) * C::LP * C::FB * TS<T>::NM;
+  static constexpr size_t FCL = FUSED ? 0 : sizeof(NS2<R>) * C::LP * C::FB * TS<T>::NM;
   static constexpr size_t FBUF =
-      FUSED ? sizeof(C2<R>) * fft_buf_elems(FftBatch<TS<T>::NLON, C::NT>::NF * TS<T>::NLON) : 0;
+      FUSED ? sizeof(C2<R>) * fft_buf_elems(C::LP * C::FB * TS<T>::NLON) : 0;
   static constexpr size_t TWS = FUSED ? sizeof(C2<R>) * TS<T>::NLON : 0;
   static constexpr size_t BYTES = FCL + FBUF + TWS;
 };
@@ -86,8 +88,8 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
   constexpr int NM = S::NM, N = S::NLON, NLAT = S::NLAT, NPB = S::NPAIR / 8;
   constexpr int LP = InvCfg<T, MODE, V>::LP, FB = InvCfg<T, MODE, V>::FB;
   constexpr int NT = InvCfg<T, MODE, V>::NT, NW = NT / 64;  // threads, waves
-  constexpr int NF = FftBatch<N, NT>::NF;
   constexpr int NTERM = (KIND == INV_UV) ? 2 : 1;
+  constexpr int NRP = N + N / 32;  // padded FFT row (fused)
 
   using L = InvLds<T, MODE, FUSED, V>;
   NS2<R>* fcl = (NS2<R>*)lds;
@@ -106,7 +108,32 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
   if (FUSED) {
     const C2<R>* tw = (const C2<R>*)twp;
     for (int k = tid; k < N; k += NT) tws[k] = tw[k];
+    // FFT inputs T+1 .. N-T-1 (above the truncation, Nyquist included) are 0
+    constexpr int NZ = N - 2 * T - 1;
+    for (int i = tid; i < LP * FB * NZ; i += NT) {
+      const int f = i / NZ, k = T + 1 + (i - f * NZ);
+      fbuf[f * NRP + k + (k >> 5)] = C2<R>{R(0), R(0)};
+    }
   }
+
+  // Fused: FFT row f = pl*FB + fl gets z_m = (n.x - s.y, n.y + s.x) and
+  // z_{N-m} = (n.x + s.y, s.x - n.y) from the northern (n) and southern (s)
+  // coefficients of wavenumber m; the imaginary parts of m = 0 are ignored,
+  // as fc2gp does.  Unfused: the coefficients as they are.
+  auto put = [&](int pl, int fl, int m, NS2<R> e) {
+    if constexpr (FUSED) {
+      C2<R>* row = fbuf + (pl * FB + fl) * NRP;
+      if (m == 0) {
+        row[0] = C2<R>{e.n.x, e.s.x};
+      } else {
+        const int kc = N - m;
+        row[m + (m >> 5)] = C2<R>{e.n.x - e.s.y, e.n.y + e.s.x};
+        row[kc + (kc >> 5)] = C2<R>{e.n.x + e.s.y, e.s.x - e.n.y};
+      }
+    } else {
+      fcl[(pl * FB + fl) * NM + m] = e;
+    }
+  };
 
   // value of MFMA/GEMM column c (0..2FB-1) for spectral index w, term t
   auto bval = [&](int term, int c, int w) -> R {
@@ -304,15 +331,33 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
           for (int i = 0; i < 4; ++i) out[i] = sa * acc1[i] + sb * p2[i];
         }
         const int fl = c >> 1, ri = c & 1;
-        if (fl < FB) {
 #pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            const int pl = 2 * q + h;
-            const float sy = out[2 * h], an = out[2 * h + 1];
-            // component ri of the northern and southern coefficient
-            R* e = (R*)&fcl[(pl * FB + fl) * NM + m];
-            e[ri] = sy + an;
-            e[2 + ri] = sy - an;
+        for (int h = 0; h < 2; ++h) {
+          const int pl = 2 * q + h;
+          const float sy = out[2 * h], an = out[2 * h + 1];
+          // component ri of the northern and southern coefficient
+          const float nv = sy + an, sv = sy - an;
+          if constexpr (FUSED) {
+            // the partner column (lane ^ 1) holds the other component: sp
+            // is s.y for ri = 0 and s.x for ri = 1
+            const float sp = __shfl_xor(sv, 1, 64);
+            if (fl < FB) {
+              R* row = (R*)(fbuf + (pl * FB + fl) * NRP);
+              if (m == 0) {
+                row[ri] = ri ? sp : nv;  // z_0 = (n.x, s.x)
+              } else {
+                const int kc = N - m;
+                // z_m = (n.x - s.y, n.y + s.x), z_{N-m} = (n.x + s.y, s.x - n.y)
+                row[2 * (m + (m >> 5)) + ri] = ri ? nv + sp : nv - sp;
+                row[2 * (kc + (kc >> 5)) + ri] = ri ? sp - nv : nv + sp;
+              }
+            }
+          } else {
+            if (fl < FB) {
+              R* e = (R*)&fcl[(pl * FB + fl) * NM + m];
+              e[ri] = nv;
+              e[2 + ri] = sv;
+            }
           }
         }
         acc1 = acc1b = acc2 = acc2b = zero4;
@@ -360,18 +405,18 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
       }
     }
   } else {
-    // vector-ALU Legendre: one (pair, column, m) sum per thread
+    // vector-ALU Legendre: one (pair, field, m) per thread, both components
     const R* tv = (const R*)tabv;
     const long tsz = (long)S::NPAIR * S::NCSP;
-    constexpr int NPC = LP * 2 * FB;
+    constexpr int NPC = LP * FB;
     const int t0 = (KIND == INV_UV) ? TQV : a.table;
     const int t1 = TQU;
     for (int idx = tid; idx < NPC * NM; idx += NT) {
       const int pc = idx % NPC, m = idx / NPC;
-      const int pl = pc % LP, c = pc / LP;
+      const int pl = pc % LP, fl = pc / LP;
       const int pair = pb * LP + pl;
       const int w0 = w0_of(T, m);
-      R sy = 0, an = 0;
+      R sy[2] = {0, 0}, an[2] = {0, 0};
       for (int n = m; n <= T; ++n) {
         const int w = w0 + n - m;
         const bool even = ((n - m) & 1) == 0;
@@ -379,22 +424,17 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
         for (int term = 0; term < NTERM; ++term) {
           const int t = term == 0 ? t0 : t1;
           const R p = tv[t * tsz + (long)pair * S::NCSP + w];
-          const R x = bval(term, c, w);
-          if (even != table_is_q(t))
-            sy += p * x;
-          else
-            an += p * x;
+#pragma unroll
+          for (int ri = 0; ri < 2; ++ri) {
+            const R x = bval(term, 2 * fl + ri, w);
+            if (even != table_is_q(t))
+              sy[ri] += p * x;
+            else
+              an[ri] += p * x;
+          }
         }
       }
-      const int fl = c >> 1, ri = c & 1;
-      NS2<R>& e = fcl[(pl * FB + fl) * NM + m];
-      if (ri == 0) {
-        e.n.x = sy + an;
-        e.s.x = sy - an;
-      } else {
-        e.n.y = sy + an;
-        e.s.y = sy - an;
-      }
+      put(pl, fl, m, NS2<R>{{sy[0] + an[0], sy[1] + an[1]}, {sy[0] - an[0], sy[1] - an[1]}});
     }
   }
   __syncthreads();
@@ -430,36 +470,20 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
       o[(long)(NLAT - 1 - lat) * N + 2 * m + 1] = vs.y;
     }
   } else {
-    // ---- phase 2: inverse FFT of z = fc_N + i fc_S, Re -> north row,
-    // Im -> south row (imag of m=0 ignored, as fc2gp does) ----
-    for (int f0 = 0; f0 < LP * FB; f0 += NF) {
-      const int nf = min(NF, LP * FB - f0);
-      // z_k = fc_N(k) + i fc_S(k) for k <= N/2 and its Hermitian extension
-      // above; zero outside the truncation and Im of m = 0 ignored (as
-      // fc2gp does).  Branch-free: one LDS read and selects per element.
-      auto load = [&](int f, int k) -> C2<R> {
-        const NS2<R>* e = fcl + (f0 + f) * NM;  // row (pair pl, field fl) = pl * FB + fl
-        const bool lo = k <= N / 2;
-        const int kk = lo ? k : N - k;
-        const bool in = kk <= T;
-        const NS2<R> v = e[in ? kk : 0];
-        const R ny = (k == 0) ? R(0) : v.n.y, sy = (k == 0) ? R(0) : v.s.y;
-        const C2<R> z = lo ? C2<R>{v.n.x - sy, ny + v.s.x} : C2<R>{v.n.x + sy, v.s.x - ny};
-        return in ? z : C2<R>{R(0), R(0)};
-      };
-      auto store = [&](int f, int k, C2<R> v) {
-        const int gi = f0 + f, pl = gi / FB, fl = gi % FB;
-        const int d = dst_of(fl);
-        if (d < 0) return;
-        const int lat = pb * LP + pl;
-        R* o = outm + (long)d * NLAT * N;
-        // grid rows stream out: non-temporal, so that they do not evict the
-        // spectral operands and tables that the other blocks reuse from L2
-        __builtin_nontemporal_store(v.x, o + (long)lat * N + k);
-        __builtin_nontemporal_store(v.y, o + (long)(NLAT - 1 - lat) * N + k);
-      };
-      fft_run<N, +1, R, NT>(fbuf, tws, nf, load, store);
-    }
+    // ---- phase 2: inverse FFT of the rows z = fc_N + i fc_S, all LP*FB at
+    // once: Re -> north row, Im -> south row ----
+    auto store = [&](int f, int k, C2<R> v) {
+      const int pl = f / FB, fl = f % FB;
+      const int d = dst_of(fl);
+      if (d < 0) return;
+      const int lat = pb * LP + pl;
+      R* o = outm + (long)d * NLAT * N;
+      // grid rows stream out: non-temporal, so that they do not evict the
+      // spectral operands and tables that the other blocks reuse from L2
+      __builtin_nontemporal_store(v.x, o + (long)lat * N + k);
+      __builtin_nontemporal_store(v.y, o + (long)(NLAT - 1 - lat) * N + k);
+    };
+    fft_run_lds<N, +1, R, NT, LP * FB, T + 1, N - T - 1>(fbuf, tws, store);
   }
 }
 

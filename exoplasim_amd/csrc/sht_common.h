@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace exo {
 
 enum Mode : int { MODE_F64 = 0, MODE_F32 = 1, MODE_F32MFMA = 2, MODE_BF16X3 = 3 };
@@ -242,12 +244,6 @@ struct FftBatch {
 __host__ __device__ constexpr int fft_buf_elems(int n) { return n + (n >> 5); }
 __device__ __forceinline__ int bix(int i) { return i + (i >> 5); }
 
-// One Stockham autosort pass (radix R, stride NS) over NF FFTs of length
-// N held in LDS.  Pass 0 reads through `load(f, k)`, the last pass writes
-// through `store(f, k, v)`; the other passes go through `buf` (NF*N
-// complex).  All 256 threads of the block must call it.  tw[k] =
-// e^{S 2 pi i k/N}.
-
 // The thread index as an opaque value.  A persistent kernel runs many work
 // items of different shapes in one loop; without this the compiler hoists
 // each item type's thread-index arithmetic out of that loop and keeps all of
@@ -258,13 +254,51 @@ __device__ __forceinline__ int item_tid() {
   return t;
 }
 
-template <int N, int R, int NS, int S, bool FIRST, bool LAST, typename F, int NT,
-          typename Load, typename Store>
+// fn(integral_constant<int, I>) for I = B .. E-1
+template <int B, int E, typename Fn>
+__device__ __forceinline__ void static_for(Fn&& fn) {
+  if constexpr (B < E) {
+    fn(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(fn);
+  }
+}
+
+// Padded index of in-row element base(j) + K over the butterflies j of a
+// pass (base(j) = j for the reads, the Stockham output index for the
+// writes): if the pad count of base(j) + K exceeds that of base(j) by the
+// same D for every j, the padded index is that of base(j) plus K + D, a
+// constant offset of one per-butterfly address; -1 otherwise.
+template <int NB, int NS, int R, int K, bool OUT>
+constexpr int pad_delta() {
+  int d = 0;
+  for (int j = 0; j < NB; ++j) {
+    const int base = OUT ? (j / NS) * NS * R + j % NS : j;
+    const int dd = ((base + K) >> 5) - (base >> 5);
+    if (j == 0)
+      d = dd;
+    else if (dd != d)
+      return -1;
+  }
+  return d;
+}
+
+// One Stockham autosort pass (radix R, stride NS) over nf (<= NFM) FFTs of
+// length N (a multiple of 32) held in LDS rows of N + N/32 complex.  With
+// FIRST the pass reads through `load(f, k)`, else from `buf`; with LAST it
+// writes through `store(f, k, v)`, else to `buf`.  The first pass (NS = 1)
+// of a transform read from `buf` treats the inputs ZLO..ZHI as zero without
+// reading them when a butterfly's whole range lies there.  All NT threads
+// of the block must call it.  tw[k] = e^{S 2 pi i k/N}.
+template <int N, int R, int NS, int S, bool FIRST, bool LAST, typename F, int NT, int NFM,
+          int ZLO, int ZHI, typename Load, typename Store>
 __device__ __forceinline__ void fft_pass(C2<F>* buf, const C2<F>* tw, int nf,
                                          Load& load, Store& store) {
+  static_assert(N % 32 == 0, "rows are padded per 32 complex");
   constexpr int NB = N / R;  // butterflies per FFT
-  constexpr int BPT = (FftBatch<N, NT>::NF * NB + NT - 1) / NT;
+  constexpr int NP = N + N / 32;
+  constexpr int BPT = (NFM * NB + NT - 1) / NT;
   const int tid = item_tid();
+  __builtin_assume(tid >= 0 && tid < NT);
   const int total = nf * NB;
   C2<F> v[BPT][R];
 #pragma unroll
@@ -272,16 +306,25 @@ __device__ __forceinline__ void fft_pass(C2<F>* buf, const C2<F>* tw, int nf,
     const int b = tid + t * NT;
     if (b < total) {
       const int f = b / NB, j = b - f * NB;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        if (FIRST)
-          v[t][r] = load(f, j + r * NB);
-        else
-          v[t][r] = buf[bix(f * N + j + r * NB)];
-      }
+      const int rb = f * NP + j + (j >> 5);
+      static_for<0, R>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        constexpr int K = r * NB;
+        constexpr int D = pad_delta<NB, NS, R, K, false>();
+        if constexpr (FIRST) {
+          v[t][r] = load(f, j + K);
+        } else if constexpr (NS == 1 && K >= ZLO && K + NB - 1 <= ZHI) {
+          v[t][r] = C2<F>{F(0), F(0)};
+        } else if constexpr (D >= 0) {
+          v[t][r] = buf[rb + K + D];
+        } else {
+          v[t][r] = buf[f * NP + j + K + ((j + K) >> 5)];
+        }
+      });
     }
   }
-  if (!FIRST) __syncthreads();
+  // in place: every read of the pass ends before the first write
+  if (!FIRST && !LAST) __syncthreads();
 #pragma unroll
   for (int t = 0; t < BPT; ++t) {
     const int b = tid + t * NT;
@@ -289,42 +332,74 @@ __device__ __forceinline__ void fft_pass(C2<F>* buf, const C2<F>* tw, int nf,
       const int f = b / NB, j = b - f * NB;
       const int jm = j % NS;
       if (NS > 1) {
+        // jm r N/(NS R) < N
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], tw[(jm * r * (N / (NS * R))) % N]);
+        for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], tw[jm * r * (N / (NS * R))]);
       }
       Dft<R, S, F>::run(v[t]);
       const int idxd = (j / NS) * NS * R + jm;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        if (LAST)
-          store(f, idxd + r * NS, v[t][r]);
+      const int wb = f * NP + idxd + (idxd >> 5);
+      static_for<0, R>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        constexpr int K = r * NS;
+        constexpr int D = pad_delta<NB, NS, R, K, true>();
+        if constexpr (LAST)
+          store(f, idxd + K, v[t][r]);
+        else if constexpr (D >= 0)
+          buf[wb + K + D] = v[t][r];
         else
-          buf[bix(f * N + idxd + r * NS)] = v[t][r];
-      }
+          buf[f * NP + idxd + K + ((idxd + K) >> 5)] = v[t][r];
+      });
     }
   }
   __syncthreads();
 }
 
 // Full FFT of nf (<= FftBatch<N, NT>::NF) sequences through the radix plan
-// by a block of NT threads.
+// by a block of NT threads: the first pass reads through `load`, the last
+// writes through `store`.
 template <int N, int S, typename F, int NT = 256, typename Load, typename Store>
 __device__ __forceinline__ void fft_run(C2<F>* buf, const C2<F>* tw, int nf, Load& load,
                                         Store& store) {
   using P = FftPlan<N>;
   constexpr int R0 = P::R[0], R1 = P::R[1], R2 = P::R[2], R3 = P::R[3];
+  constexpr int M = FftBatch<N, NT>::NF;
   if constexpr (P::NP == 2) {
-    fft_pass<N, R0, 1, S, true, false, F, NT>(buf, tw, nf, load, store);
-    fft_pass<N, R1, R0, S, false, true, F, NT>(buf, tw, nf, load, store);
+    fft_pass<N, R0, 1, S, true, false, F, NT, M, 1, 0>(buf, tw, nf, load, store);
+    fft_pass<N, R1, R0, S, false, true, F, NT, M, 1, 0>(buf, tw, nf, load, store);
   } else if constexpr (P::NP == 3) {
-    fft_pass<N, R0, 1, S, true, false, F, NT>(buf, tw, nf, load, store);
-    fft_pass<N, R1, R0, S, false, false, F, NT>(buf, tw, nf, load, store);
-    fft_pass<N, R2, R0 * R1, S, false, true, F, NT>(buf, tw, nf, load, store);
+    fft_pass<N, R0, 1, S, true, false, F, NT, M, 1, 0>(buf, tw, nf, load, store);
+    fft_pass<N, R1, R0, S, false, false, F, NT, M, 1, 0>(buf, tw, nf, load, store);
+    fft_pass<N, R2, R0 * R1, S, false, true, F, NT, M, 1, 0>(buf, tw, nf, load, store);
   } else {
-    fft_pass<N, R0, 1, S, true, false, F, NT>(buf, tw, nf, load, store);
-    fft_pass<N, R1, R0, S, false, false, F, NT>(buf, tw, nf, load, store);
-    fft_pass<N, R2, R0 * R1, S, false, false, F, NT>(buf, tw, nf, load, store);
-    fft_pass<N, R3, R0 * R1 * R2, S, false, true, F, NT>(buf, tw, nf, load, store);
+    fft_pass<N, R0, 1, S, true, false, F, NT, M, 1, 0>(buf, tw, nf, load, store);
+    fft_pass<N, R1, R0, S, false, false, F, NT, M, 1, 0>(buf, tw, nf, load, store);
+    fft_pass<N, R2, R0 * R1, S, false, false, F, NT, M, 1, 0>(buf, tw, nf, load, store);
+    fft_pass<N, R3, R0 * R1 * R2, S, false, true, F, NT, M, 1, 0>(buf, tw, nf, load, store);
+  }
+}
+
+// Full FFT of NFM sequences already in `buf` (rows of N + N/32 complex),
+// in place until the last pass, which writes through `store`.  Inputs
+// ZLO..ZHI of every row are zero (and must be zero in `buf`).
+template <int N, int S, typename F, int NT, int NFM, int ZLO, int ZHI, typename Store>
+__device__ __forceinline__ void fft_run_lds(C2<F>* buf, const C2<F>* tw, Store& store) {
+  using P = FftPlan<N>;
+  constexpr int R0 = P::R[0], R1 = P::R[1], R2 = P::R[2], R3 = P::R[3];
+  auto none = [](int, int) { return C2<F>{F(0), F(0)}; };
+  if constexpr (P::NP == 2) {
+    fft_pass<N, R0, 1, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
+    fft_pass<N, R1, R0, S, false, true, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
+  } else if constexpr (P::NP == 3) {
+    fft_pass<N, R0, 1, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
+    fft_pass<N, R1, R0, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
+    fft_pass<N, R2, R0 * R1, S, false, true, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
+  } else {
+    fft_pass<N, R0, 1, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
+    fft_pass<N, R1, R0, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
+    fft_pass<N, R2, R0 * R1, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
+    fft_pass<N, R3, R0 * R1 * R2, S, false, true, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none,
+                                                                       store);
   }
 }
 
