@@ -321,10 +321,9 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
         if (KIND == INV_UV) {
           // u = qv.zeta + qu.(-i D), v = qu.(-i zeta) - qv.D: the qu product of
           // the partner column (zeta.re <-> D.im, zeta.im <-> D.re)
-          const int plane_ = (lane & ~3) | (3 - (lane & 3));
           f32x4 p2;
 #pragma unroll
-          for (int i = 0; i < 4; ++i) p2[i] = __shfl(acc2[i], plane_, 64);
+          for (int i = 0; i < 4; ++i) p2[i] = quad_perm<0x1B>(acc2[i]);  // lane ^ 3
           // column c & 3 = 0: acc1 + p2, 1: acc1 - p2, 2: p2 - acc1, 3: -p2 - acc1
           const float sa = (c & 2) ? -1.f : 1.f, sb = (c & 1) ? -1.f : 1.f;
 #pragma unroll
@@ -340,7 +339,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
           if constexpr (FUSED) {
             // the partner column (lane ^ 1) holds the other component: sp
             // is s.y for ri = 0 and s.x for ri = 1
-            const float sp = __shfl_xor(sv, 1, 64);
+            const float sp = quad_perm<0xB1>(sv);  // lane ^ 1
             if (fl < FB) {
               R* row = (R*)(fbuf + (pl * FB + fl) * NRP);
               if (m == 0) {
@@ -471,19 +470,30 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     }
   } else {
     // ---- phase 2: inverse FFT of the rows z = fc_N + i fc_S, all LP*FB at
-    // once: Re -> north row, Im -> south row ----
-    auto store = [&](int f, int k, C2<R> v) {
+    // once and in place: Re -> north row, Im -> south row ----
+    auto none = [](int, int, C2<R>) {};
+    fft_run_lds<N, +1, R, NT, LP * FB, T + 1, N - T - 1, false>(fbuf, tws, none);
+    // the grid rows leave in runs of 4 values per lane, so that a wave
+    // writes whole rows: non-temporal, so that they do not evict the
+    // spectral operands and tables that the other blocks reuse from L2
+    typedef R R4 __attribute__((ext_vector_type(4)));
+    constexpr int NK4 = N / 4;
+    for (int i = tid; i < LP * FB * NK4; i += NT) {
+      const int f = i / NK4, k = (i - f * NK4) * 4;
       const int pl = f / FB, fl = f % FB;
       const int d = dst_of(fl);
-      if (d < 0) return;
+      if (d < 0) continue;
+      // k .. k+3 lie in one 32-element run of the padded row
+      const C2<R>* z = fbuf + f * NRP + k + (k >> 5);
+      const C2<R> z0 = z[0], z1 = z[1], z2 = z[2], z3 = z[3];
       const int lat = pb * LP + pl;
-      R* o = outm + (long)d * NLAT * N;
-      // grid rows stream out: non-temporal, so that they do not evict the
-      // spectral operands and tables that the other blocks reuse from L2
-      __builtin_nontemporal_store(v.x, o + (long)lat * N + k);
-      __builtin_nontemporal_store(v.y, o + (long)(NLAT - 1 - lat) * N + k);
-    };
-    fft_run_lds<N, +1, R, NT, LP * FB, T + 1, N - T - 1>(fbuf, tws, store);
+      R* o = outm + (long)d * NLAT * N + k;
+      __builtin_nontemporal_store(R4{z0.x, z1.x, z2.x, z3.x}, (R4*)(o + (long)lat * N));
+      __builtin_nontemporal_store(R4{z0.y, z1.y, z2.y, z3.y},
+                                  (R4*)(o + (long)(NLAT - 1 - lat) * N));
+    }
+    // the rows are read before a next work item of the block rewrites them
+    __syncthreads();
   }
 }
 

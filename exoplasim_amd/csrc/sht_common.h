@@ -90,6 +90,13 @@ __device__ __forceinline__ uint4 buffer_load_x4(__amdgpu_buffer_rsrc_t r, int vo
   const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
   return uint4{v[0], v[1], v[2], v[3]};
 }
+// x from another lane of the same group of four (DPP quad permutation,
+// CTRL = p0 | p1 << 2 | p2 << 4 | p3 << 6): no LDS round trip
+template <int CTRL>
+__device__ __forceinline__ float quad_perm(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+
 // a per-lane byte offset that no buffer covers: such a load returns 0
 constexpr int BUF_NONE = 0x7ff00000;
 
@@ -380,26 +387,27 @@ __device__ __forceinline__ void fft_run(C2<F>* buf, const C2<F>* tw, int nf, Loa
 }
 
 // Full FFT of NFM sequences already in `buf` (rows of N + N/32 complex),
-// in place until the last pass, which writes through `store`.  Inputs
+// in place; with OUT the last pass writes through `store` instead.  Inputs
 // ZLO..ZHI of every row are zero (and must be zero in `buf`).
-template <int N, int S, typename F, int NT, int NFM, int ZLO, int ZHI, typename Store>
+template <int N, int S, typename F, int NT, int NFM, int ZLO, int ZHI, bool OUT = true,
+          typename Store>
 __device__ __forceinline__ void fft_run_lds(C2<F>* buf, const C2<F>* tw, Store& store) {
   using P = FftPlan<N>;
   constexpr int R0 = P::R[0], R1 = P::R[1], R2 = P::R[2], R3 = P::R[3];
   auto none = [](int, int) { return C2<F>{F(0), F(0)}; };
   if constexpr (P::NP == 2) {
     fft_pass<N, R0, 1, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
-    fft_pass<N, R1, R0, S, false, true, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
+    fft_pass<N, R1, R0, S, false, OUT, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
   } else if constexpr (P::NP == 3) {
     fft_pass<N, R0, 1, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
     fft_pass<N, R1, R0, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
-    fft_pass<N, R2, R0 * R1, S, false, true, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
+    fft_pass<N, R2, R0 * R1, S, false, OUT, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
   } else {
     fft_pass<N, R0, 1, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
     fft_pass<N, R1, R0, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
     fft_pass<N, R2, R0 * R1, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
-    fft_pass<N, R3, R0 * R1 * R2, S, false, true, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none,
-                                                                       store);
+    fft_pass<N, R3, R0 * R1 * R2, S, false, OUT, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none,
+                                                                      store);
   }
 }
 
