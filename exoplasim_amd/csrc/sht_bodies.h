@@ -215,7 +215,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     if constexpr (MODE == MODE_BF16X3) {
       rsB = buffer_rsrc((const uint16_t*)a.bpk + mem * a.bpk_stride, (int)a.bpk_stride * 2);
       if (pf >= 0)
-        bvoff = ((pf - a.bpk_f0) * (pk_blocks(T) * PK_BLK) + (c & 1) * 32 + q * 8) * 2;
+        bvoff = ((pf - a.bpk_f0) * pk_field(T) + q * 8) * 2;
     }
     const R pv = (R)a.plavor;
     // B fragment of one item: 8 fp32 values (F32MFMA, n = 4j + q) or the
@@ -226,9 +226,12 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     };
     auto fetch = [&](int m, int kb, BFrag& br, uint4* a0, uint4* a1) {
       if constexpr (MODE == MODE_BF16X3) {
-        const int bso = ((pk_blkoff(T, m) + kb) * PK_BLK) * 2;
+        // segment of m: plane rows of 2*p8 values, re then im
+        const int p8 = pk_p8(T + 1 - m);
+        const int bso = (pk_seg(T, m) + 32 * kb) * 2;
+        const int bv = bvoff + (c & 1) * p8 * 2;
 #pragma unroll
-        for (int p = 0; p < 3; ++p) br.p[p].q = buffer_load_x4(rsB, bvoff, bso + p * 128);
+        for (int p = 0; p < 3; ++p) br.p[p].q = buffer_load_x4(rsB, bv, bso + p * p8 * 4);
         if (m == 0 && kb == 0 && isvor && q == 0) {
           // absolute -> relative vorticity at (m=0, n=1, re): the split is
           // exact, so the sum of the planes is the fp32 coefficient

@@ -225,7 +225,7 @@ static int pack_for(exo_sht_plan_t p, int mode, int nmem, const void* sp, long s
   if (launch_pack(p->tb, (const float*)sp, sps, f0, nf, nmem, (uint16_t*)p->pk, s) != 0)
     return EXO_ERR_LAUNCH;
   a.bpk = p->pk;
-  a.bpk_stride = (long)nf * pk_blocks(p->tb.ntru) * PK_BLK;
+  a.bpk_stride = (long)nf * pk_field(p->tb.ntru);
   a.bpk_f0 = f0;
   return EXO_OK;
 }

@@ -59,17 +59,19 @@ __device__ __forceinline__ int m_of_w(int ntru, int w) {
 }
 
 // Pre-split spectral operand of the split-BF16 inverse transform.  Per
-// field, for each zonal wavenumber m and each 32-wide block kb of total
-// wavenumbers n = m + 32 kb + i: [plane hi/mid/lo][re/im][32 n] bf16, zero
-// for n > NTRU.  pk_cum(x) = sum_{a=1..x} ceil(a/32), so the block of (m, kb)
-// starts at block pk_blkoff(m) + kb and a field holds pk_cum(NTRU+1) blocks.
-constexpr int PK_BLK = 3 * 2 * 32;  // bf16 values per (m, kb) block
-__host__ __device__ constexpr int pk_cum(int x) {
-  return 16 * (x / 32) * (x / 32 + 1) + (x % 32) * (x / 32 + 1);
+// field, for each zonal wavenumber m, one segment [plane hi/mid/lo][re/im]
+// [pk_p8(L)] of bf16 holding n = m .. NTRU (L = NTRU + 1 - m values, zero
+// padded to a multiple of 8).  pk_c8(x) = sum_{L=1..x} pk_p8(L), so the
+// segment of m starts at pk_seg(m) and a field holds pk_field() values.
+// A 32-wide block of n read from a segment may run into the next segment:
+// those n exceed NTRU, where the table operand is zero.
+__host__ __device__ constexpr int pk_p8(int L) { return (L + 7) & ~7; }
+__host__ __device__ constexpr int pk_c8(int x) {
+  return 32 * (x / 8) * (x / 8 + 1) + 8 * (x % 8) * (x / 8 + 1);
 }
-__host__ __device__ constexpr int pk_blocks(int ntru) { return pk_cum(ntru + 1); }
-__host__ __device__ constexpr int pk_blkoff(int ntru, int m) {
-  return pk_cum(ntru + 1) - pk_cum(ntru + 1 - m);
+__host__ __device__ constexpr int pk_field(int ntru) { return 6 * pk_c8(ntru + 1); }
+__host__ __device__ constexpr int pk_seg(int ntru, int m) {
+  return 6 * (pk_c8(ntru + 1) - pk_c8(ntru + 1 - m));
 }
 
 // Raw buffer loads: the per-lane byte offset in a VGPR and a wave-uniform

@@ -263,7 +263,7 @@ static int ens_prepare(exo_sht_ens_s* e) {
       j.a.var = e->inv_var[k];
       if (e->mode == MODE_BF16X3) {
         j.a.bpk = e->pk[k];
-        j.a.bpk_stride = (long)nfl * pk_blocks(tb.ntru) * PK_BLK;
+        j.a.bpk_stride = (long)nfl * pk_field(tb.ntru);
         j.a.bpk_f0 = 0;
       }
       int t0 = TQI;

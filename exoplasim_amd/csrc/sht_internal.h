@@ -98,11 +98,11 @@ size_t forward_scratch_bytes(const Tables& T, int mode, int nmem, int nfield);
 int launch_fft(const Tables& T, int mode, int dir, void* a, int n, long lot, hipStream_t s);
 // pre-split copy (PK layout, sht_common.h) of fp32 spectral fields [f0, f0+nf)
 // of nmem members for the split-BF16 inverse; out holds nmem * nf *
-// pk_blocks(ntru) * PK_BLK bf16 values
+// pk_field(ntru) bf16 values
 int launch_pack(const Tables& T, const float* sp, long sp_stride, int f0, int nf, int nmem,
                 uint16_t* out, hipStream_t s);
 inline size_t pack_bytes(const Tables& T, int nmem, int nf) {
-  return sizeof(uint16_t) * (size_t)nmem * nf * pk_blocks(T.ntru) * PK_BLK;
+  return sizeof(uint16_t) * (size_t)nmem * nf * pk_field(T.ntru);
 }
 
 // the device tables of a plan (defined with the plan object)

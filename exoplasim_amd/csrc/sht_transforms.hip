@@ -148,49 +148,50 @@ static int fft_t(const void* tw, int dir, R* a, long lot, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // pre-split spectral operand of the split-BF16 inverse (layout in
-// sht_common.h): one thread per 8 consecutive n of one (member, field, m, kb,
-// re/im); the three planes are written as 16-byte chunks, eight threads to a
-// contiguous 128-byte plane row
+// sht_common.h): one thread per 8 consecutive n of one (member, field, m,
+// re/im) row; each of the three planes is written as one 16-byte chunk, so
+// consecutive threads fill a plane row contiguously
 // ---------------------------------------------------------------------------
 template <int T>
 __global__ __launch_bounds__(256) void k_pack_sp(const float* sp, long sp_stride, int f0, int nf,
                                                  int nmem, uint16_t* out) {
   using S = TS<T>;
-  constexpr int NB = pk_blocks(T);
+  constexpr int NC = pk_c8(T + 1) / 4;  // 8-value chunks per field (re and im)
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= (long)nmem * nf * NB * 8) return;
-  const int ri = (int)(i >> 2) & 1, q = (int)i & 3;
-  const long bi = i >> 3;  // (member, field, block)
-  const int blk = (int)(bi % NB);
-  const long fm = bi / NB;
+  if (i >= (long)nmem * nf * NC) return;
+  const int u = (int)(i % NC);
+  const long fm = i / NC;
   const int field = (int)(fm % nf), mem = (int)(fm / nf);
-  int lo = 0, hi = T;  // the largest m whose first block is <= blk
+  // the largest m whose segment starts at or before chunk u
+  int lo = 0, hi = T;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (pk_blkoff(T, mid) <= blk)
+    if (pk_seg(T, mid) / 24 <= u)
       lo = mid;
     else
       hi = mid - 1;
   }
-  const int m = lo, kb = blk - pk_blkoff(T, m);
+  const int m = lo, p8 = pk_p8(T + 1 - m);
+  const int r = u - pk_seg(T, m) / 24;  // chunk within the segment's re/im rows
+  const int ri = r / (p8 / 8), dn0 = (r - ri * (p8 / 8)) * 8;
   const float* src = sp + mem * sp_stride + (long)(f0 + field) * S::NRSP + 2 * w0_of(T, m) + ri;
   BF8 h, md, l;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    const int dn = kb * 32 + q * 8 + j;  // n - m
+    const int dn = dn0 + j;  // n - m
     const float x = (m + dn <= T) ? src[2 * dn] : 0.f;
     split3(x, h.s[j], md.s[j], l.s[j]);
   }
-  uint16_t* dst = out + bi * PK_BLK + ri * 32 + q * 8;
+  uint16_t* dst = out + fm * pk_field(T) + pk_seg(T, m) + ri * p8 + dn0;
   *(uint4*)dst = h.q;
-  *(uint4*)(dst + 64) = md.q;
-  *(uint4*)(dst + 128) = l.q;
+  *(uint4*)(dst + 2 * p8) = md.q;
+  *(uint4*)(dst + 4 * p8) = l.q;
 }
 
 template <int T>
 static int pack_t(const float* sp, long sps, int f0, int nf, int nmem, uint16_t* out,
                   hipStream_t s) {
-  const long total = (long)nmem * nf * pk_blocks(T) * 8;
+  const long total = (long)nmem * nf * (pk_c8(T + 1) / 4);
   if (total <= 0) return 0;
   k_pack_sp<T><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(sp, sps, f0, nf, nmem, out);
   return hipGetLastError() == hipSuccess ? 0 : -3;
