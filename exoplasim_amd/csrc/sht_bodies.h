@@ -196,49 +196,67 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     if constexpr (MODE == MODE_F32MFMA) {
       if (pf >= 0) bp = spm + (long)pf * S::NRSP + (c & 1);
     }
-    // Operands through buffer loads: a per-lane byte offset (loop-invariant)
-    // plus a wave-uniform byte offset per item.
-    // A: the tables of pair block pb; item (m, kb) at ((m KB + kb) 512)
-    // elements, planes `plane` elements apart.
+    // Operands through buffer loads: a per-lane byte offset plus a
+    // wave-uniform byte offset per item.
+    // BF16X3: the K index of lane quarter q is parity h = q >> 1, row
+    // element j = 16 kb + 8 (q & 1) .. + 7 (layouts in sht_common.h).
+    // A: the table rows of pair block pb; a lane whose parity is not its
+    // row's class, or whose j lies past the row, reads past the buffer,
+    // which returns the zeros.  F32MFMA: item (m, kb) at ((m KB + kb) 512)
+    // floats of the block's table.
     constexpr int AE = (MODE == MODE_BF16X3) ? 2 : 4;  // bytes per table element
-    const long abase = (long)pb * NM * S::KB * 512;
-    const int abytes = (int)(((MODE == MODE_BF16X3) ? 3 : 1) * plane - abase) * AE;
+    const long abase = (MODE == MODE_BF16X3) ? (long)pb * tab_pb(T) : (long)pb * NM * S::KB * 512;
+    const int abytes = (MODE == MODE_BF16X3) ? tab_pb(T) * 2 : (int)(plane - abase) * 4;
     const __amdgpu_buffer_rsrc_t rsA0 = buffer_rsrc((const char*)tabA0 + abase * AE, abytes);
     const __amdgpu_buffer_rsrc_t rsA1 =
         buffer_rsrc((const char*)(NTERM == 2 ? tabA1 : tabA0) + abase * AE, abytes);
-    const int avoff = (MODE == MODE_BF16X3) ? (c * 32 + q * 8) * 2 : (q * 16 + c) * 32;
-    const int aplane = (int)plane * AE;
+    const int hq = q >> 1, jq = 8 * (q & 1);
+    // does this lane hold nonzero table values (its parity is its row's class)
+    const bool amine0 = hq == ((c & 1) ^ (table_is_q(KIND == INV_UV ? TQV : a.table) ? 1 : 0));
+    const bool amine1 = hq == ((c & 1) ^ (table_is_q(TQU) ? 1 : 0));
+    const int avoff = (q * 16 + c) * 32;  // F32MFMA
     // B (BF16X3): the member's pre-split operand; columns without a field
     // read past the buffer, which returns zeros
     __amdgpu_buffer_rsrc_t rsB = rsA0;
     int bvoff = BUF_NONE;
     if constexpr (MODE == MODE_BF16X3) {
       rsB = buffer_rsrc((const uint16_t*)a.bpk + mem * a.bpk_stride, (int)a.bpk_stride * 2);
-      if (pf >= 0)
-        bvoff = ((pf - a.bpk_f0) * pk_field(T) + q * 8) * 2;
+      if (pf >= 0) bvoff = ((pf - a.bpk_f0) * pk_field(T) + jq) * 2;
     }
+    const int brow = (c & 1) * 2 + hq;  // re/im, parity row of the B segment
     const R pv = (R)a.plavor;
     // B fragment of one item: 8 fp32 values (F32MFMA, n = 4j + q) or the
-    // three bf16 planes of 8 consecutive n (BF16X3, n = 8q + j)
+    // three bf16 planes of 8 values of one parity (BF16X3)
     struct BFrag {
       float f[8];
       BF8 p[3];
     };
     auto fetch = [&](int m, int kb, BFrag& br, uint4* a0, uint4* a1) {
       if constexpr (MODE == MODE_BF16X3) {
-        // segment of m: plane rows of 2*p8 values, re then im
-        const int p8 = pk_p8(T + 1 - m);
-        const int bso = (pk_seg(T, m) + 32 * kb) * 2;
-        const int bv = bvoff + (c & 1) * p8 * 2;
+        const int rw = pk_rw(T, m);
+        const int bso = (pk_seg(T, m) + 16 * kb) * 2;
+        const int bv = bvoff + brow * rw * 2;
 #pragma unroll
-        for (int p = 0; p < 3; ++p) br.p[p].q = buffer_load_x4(rsB, bv, bso + p * p8 * 4);
-        if (m == 0 && kb == 0 && isvor && q == 0) {
-          // absolute -> relative vorticity at (m=0, n=1, re): the split is
-          // exact, so the sum of the planes is the fp32 coefficient
-          const float x = __uint_as_float((uint32_t)br.p[0].s[1] << 16) +
-                          __uint_as_float((uint32_t)br.p[1].s[1] << 16) +
-                          __uint_as_float((uint32_t)br.p[2].s[1] << 16);
-          split3(x - (float)pv, br.p[0].s[1], br.p[1].s[1], br.p[2].s[1]);
+        for (int p = 0; p < 3; ++p) br.p[p].q = buffer_load_x4(rsB, bv, bso + p * rw * 8);
+        if (m == 0 && kb == 0 && isvor && q == 2) {
+          // absolute -> relative vorticity at (m=0, n=1, re), the first odd
+          // value: the split is exact, so the sum of the planes is the fp32
+          // coefficient
+          const float x = __uint_as_float((uint32_t)br.p[0].s[0] << 16) +
+                          __uint_as_float((uint32_t)br.p[1].s[0] << 16) +
+                          __uint_as_float((uint32_t)br.p[2].s[0] << 16);
+          split3(x - (float)pv, br.p[0].s[0], br.p[1].s[0], br.p[2].s[0]);
+        }
+        const int aso = (tab_seg(T, m) + 16 * kb) * 2;
+        const bool inrow = 16 * kb + jq < rw;
+        const int av = (c * rw + jq) * 2;
+        const int av0 = (amine0 && inrow) ? av : BUF_NONE;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) a0[p] = buffer_load_x4(rsA0, av0, aso + p * rw * 32);
+        if (NTERM == 2) {
+          const int av1 = (amine1 && inrow) ? av : BUF_NONE;
+#pragma unroll
+          for (int p = 0; p < 3; ++p) a1[p] = buffer_load_x4(rsA1, av1, aso + p * rw * 32);
         }
       } else {
         const int w0 = w0_of(T, m);
@@ -252,17 +270,8 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
           }
           br.f[j] = (float)x;
         }
-      }
-      const int aso = (m * S::KB + kb) * 512 * AE;
-      if constexpr (MODE == MODE_BF16X3) {
-#pragma unroll
-        for (int p = 0; p < 3; ++p) a0[p] = buffer_load_x4(rsA0, avoff, aso + p * aplane);
-        if (NTERM == 2) {
-#pragma unroll
-          for (int p = 0; p < 3; ++p) a1[p] = buffer_load_x4(rsA1, avoff, aso + p * aplane);
-        }
-      } else {
         // f32 operand stored as [q][row][8] so one lane reads 8 floats
+        const int aso = (m * S::KB + kb) * 512 * AE;
         a0[0] = buffer_load_x4(rsA0, avoff, aso);
         a0[1] = buffer_load_x4(rsA0, avoff, aso + 16);
         if (NTERM == 2) {

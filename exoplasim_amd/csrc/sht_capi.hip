@@ -114,11 +114,15 @@ static int build_tables(Tables& t, int ntru, const double* sid, const double* gw
   t.inv_elems = (long)(t.npair / 8) * (ntru + 1) * t.kbmax * 512;
   const int inv_tabs[] = {TQI, TQJ, TQU, TQV};
   for (int tt : inv_tabs) {
-    HIPCK(hipMalloc(&t.inv_bf[tt], sizeof(uint16_t) * 3 * t.inv_elems));
+    const long nbf = (long)(t.npair / 8) * tab_pb(ntru);
+    HIPCK(hipMalloc(&t.inv_bf[tt], sizeof(uint16_t) * nbf));
     HIPCK(hipMalloc(&t.inv_f32[tt], sizeof(float) * t.inv_elems));
     k_pack_inv<<<(unsigned)((t.inv_elems + 255) / 256), 256>>>(ntru, t.npair, t.kbmax, tt,
-                                                              t.filt64, t.inv_bf[tt],
-                                                              t.inv_f32[tt]);
+                                                              t.filt64, t.inv_f32[tt]);
+    HIPCK(hipGetLastError());
+    const long nth = nbf / 3;
+    k_pack_inv_bf<<<(unsigned)((nth + 255) / 256), 256>>>(ntru, t.npair, tt, t.filt64,
+                                                          t.inv_bf[tt]);
     HIPCK(hipGetLastError());
   }
   k_twiddle<<<(t.nlon + 255) / 256, 256>>>(t.nlon, t.tw64, t.tw32);

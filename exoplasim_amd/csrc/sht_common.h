@@ -58,20 +58,29 @@ __device__ __forceinline__ int m_of_w(int ntru, int w) {
   return m;
 }
 
-// Pre-split spectral operand of the split-BF16 inverse transform.  Per
-// field, for each zonal wavenumber m, one segment [plane hi/mid/lo][re/im]
-// [pk_p8(L)] of bf16 holding n = m .. NTRU (L = NTRU + 1 - m values, zero
-// padded to a multiple of 8).  pk_c8(x) = sum_{L=1..x} pk_p8(L), so the
-// segment of m starts at pk_seg(m) and a field holds pk_field() values.
-// A 32-wide block of n read from a segment may run into the next segment:
-// those n exceed NTRU, where the table operand is zero.
-__host__ __device__ constexpr int pk_p8(int L) { return (L + 7) & ~7; }
-__host__ __device__ constexpr int pk_c8(int x) {
-  return 32 * (x / 8) * (x / 8 + 1) + 8 * (x % 8) * (x / 8 + 1);
+// Operands of the split-BF16 inverse Legendre GEMM.  The reduction runs
+// over total wavenumbers n in 32-wide blocks kb per zonal wavenumber m,
+// each block in parity order: k = j holds n - m = 32 kb + 2 j and
+// k = 16 + j holds n - m = 32 kb + 2 j + 1 (j = 0..15).  Both operands are
+// stored as rows of one parity h: element j of a row is n - m = 2 j + h,
+// pk_rw(m) = 8 ceil((NTRU + 1 - m) / 16) values per row, zero for n > NTRU.
+// pk_c16(x) = sum_{L=1..x} 8 ceil(L/16) sums the row lengths.
+__host__ __device__ constexpr int pk_rw(int ntru, int m) { return 8 * ((ntru + 16 - m) / 16); }
+__host__ __device__ constexpr int pk_c16(int x) {
+  return 64 * (x / 16) * (x / 16 + 1) + 8 * (x % 16) * (x / 16 + 1);
 }
-__host__ __device__ constexpr int pk_field(int ntru) { return 6 * pk_c8(ntru + 1); }
+// spectral operand: per field, for each m, [plane hi/mid/lo][re/im][h][pk_rw]
+__host__ __device__ constexpr int pk_field(int ntru) { return 12 * pk_c16(ntru + 1); }
 __host__ __device__ constexpr int pk_seg(int ntru, int m) {
-  return 6 * (pk_c8(ntru + 1) - pk_c8(ntru + 1 - m));
+  return 12 * (pk_c16(ntru + 1) - pk_c16(ntru + 1 - m));
+}
+// table operand: per block of 8 latitude pairs, for each m,
+// [plane][row = 2 pair + class][pk_rw]; the row holds the parity of its
+// class (h = class for P-type tables, 1 - class for Q-type), the other
+// parity being zero and not stored
+__host__ __device__ constexpr int tab_pb(int ntru) { return 48 * pk_c16(ntru + 1); }
+__host__ __device__ constexpr int tab_seg(int ntru, int m) {
+  return 48 * (pk_c16(ntru + 1) - pk_c16(ntru + 1 - m));
 }
 
 // Raw buffer loads: the per-lane byte offset in a VGPR and a wave-uniform

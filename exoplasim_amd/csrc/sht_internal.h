@@ -14,7 +14,8 @@ __global__ void k_legini(int ntru, int npair, const double* sid, const double* g
 __global__ void k_filtered(int ntru, int npair, const double* tab, const double* skgpsp,
                            const double* skspgp, double* out64, float* out32);
 __global__ void k_pack_inv(int ntru, int npair, int kbmax, int t, const double* filt64,
-                           uint16_t* bf, float* f32);
+                           float* f32);
+__global__ void k_pack_inv_bf(int ntru, int npair, int t, const double* filt64, uint16_t* bf);
 __global__ void k_twiddle(int n, double* tw64, float* tw32);
 
 // Which inverse transform a launch performs.
@@ -34,9 +35,9 @@ struct Tables {
   float* filt32 = nullptr;
   double* skgpsp = nullptr;   // [ntru+1]
   double* skspgp = nullptr;
-  uint16_t* inv_bf[NTAB] = {};  // packed inverse A operands, 3 planes
-  float* inv_f32[NTAB] = {};
-  long inv_elems = 0;
+  uint16_t* inv_bf[NTAB] = {};  // split-BF16 inverse table operands (3 planes)
+  float* inv_f32[NTAB] = {};    // FP32-MFMA inverse table operands
+  long inv_elems = 0;           // elements of one inv_f32 table
   double* tw64 = nullptr;     // [nlon] complex
   float* tw32 = nullptr;
   double plavor = 0.0;

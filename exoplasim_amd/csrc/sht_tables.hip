@@ -149,10 +149,12 @@ __device__ void split3_f64(double x, uint16_t& h, uint16_t& m, uint16_t& l) {
   l = (uint16_t)(ul >> 16);
 }
 
-// ---- inverse A operand: [pb][m][kb][row 16][k 32], row = 2*pair_local + cls,
-// k = n - m - 32 kb; zero where n > NTRU or the class does not match ----
+// ---- table operand of the FP32-MFMA inverse: [pb][m][kb][512] with row =
+// 2*pair_local + cls and k = n - m - 32 kb, stored [k%4][row][k/4] per
+// 512-element block so that one lane of v_mfma_f32_16x16x4_f32 reads its 8
+// values contiguously; zero where n > NTRU or the class does not match ----
 __global__ void k_pack_inv(int ntru, int npair, int kbmax, int t, const double* filt64,
-                           uint16_t* bf, float* f32) {
+                           float* f32) {
   const int ncsp = (ntru + 1) * (ntru + 2) / 2;
   const long tsz = (long)npair * ncsp;
   const int npb = npair / 8;
@@ -172,15 +174,41 @@ __global__ void k_pack_inv(int ntru, int npair, int kbmax, int t, const double* 
     const int mycls = (even != table_is_q(t)) ? 0 : 1;
     if (mycls == cls) v = filt64[t * tsz + (long)(pb * 8 + pl) * ncsp + w0_of(ntru, m) + (n - m)];
   }
-  uint16_t h, mm, l;
-  split3_f64(v, h, mm, l);
-  bf[i] = h;
-  bf[total + i] = mm;
-  bf[2 * total + i] = l;
-  // the f32 operand of v_mfma_f32_16x16x4_f32 is stored [k%4][row][k/4] per
-  // 512-element block so that one lane reads its 8 values contiguously
   const long blk = i - (i % 512);
   f32[blk + ((k % 4) * 16 + row) * 8 + k / 4] = (float)v;
+}
+
+// ---- table operand of the split-BF16 inverse (layout in sht_common.h):
+// one thread per (pair block, m, row, j) writing the three planes ----
+__global__ void k_pack_inv_bf(int ntru, int npair, int t, const double* filt64, uint16_t* bf) {
+  const int ncsp = (ntru + 1) * (ntru + 2) / 2;
+  const long tsz = (long)npair * ncsp;
+  const int per = 16 * pk_c16(ntru + 1);  // (row, j) per pair block
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= (long)(npair / 8) * per) return;
+  const int pb = (int)(i / per), e = (int)(i % per);
+  int lo = 0, hi = ntru;  // the largest m whose segment starts at or before e
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab_seg(ntru, mid) / 3 <= e)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int m = lo, rw = pk_rw(ntru, m);
+  const int r = e - tab_seg(ntru, m) / 3;
+  const int row = r / rw, j = r - row * rw;
+  const int pl = row >> 1, cls = row & 1;
+  const int h = cls ^ (table_is_q(t) ? 1 : 0);
+  const int n = m + 2 * j + h;
+  const double v =
+      (n <= ntru) ? filt64[t * tsz + (long)(pb * 8 + pl) * ncsp + w0_of(ntru, m) + (n - m)] : 0.0;
+  uint16_t hh, mm, ll;
+  split3_f64(v, hh, mm, ll);
+  uint16_t* o = bf + (long)pb * tab_pb(ntru) + tab_seg(ntru, m) + row * rw + j;
+  o[0] = hh;
+  o[16 * rw] = mm;
+  o[32 * rw] = ll;
 }
 
 // twiddles e^{+2 pi i k/N} in FP64 and FP32

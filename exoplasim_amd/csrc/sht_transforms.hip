@@ -148,15 +148,15 @@ static int fft_t(const void* tw, int dir, R* a, long lot, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // pre-split spectral operand of the split-BF16 inverse (layout in
-// sht_common.h): one thread per 8 consecutive n of one (member, field, m,
-// re/im) row; each of the three planes is written as one 16-byte chunk, so
-// consecutive threads fill a plane row contiguously
+// sht_common.h): one thread per 8 consecutive values of one (member, field,
+// m, re/im, parity) row; each of the three planes is written as one 16-byte
+// chunk, so consecutive threads fill a plane row contiguously
 // ---------------------------------------------------------------------------
 template <int T>
 __global__ __launch_bounds__(256) void k_pack_sp(const float* sp, long sp_stride, int f0, int nf,
                                                  int nmem, uint16_t* out) {
   using S = TS<T>;
-  constexpr int NC = pk_c8(T + 1) / 4;  // 8-value chunks per field (re and im)
+  constexpr int NC = pk_c16(T + 1) / 2;  // 8-value chunks per field (4 rows per m)
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i >= (long)nmem * nf * NC) return;
   const int u = (int)(i % NC);
@@ -171,27 +171,28 @@ __global__ __launch_bounds__(256) void k_pack_sp(const float* sp, long sp_stride
     else
       hi = mid - 1;
   }
-  const int m = lo, p8 = pk_p8(T + 1 - m);
-  const int r = u - pk_seg(T, m) / 24;  // chunk within the segment's re/im rows
-  const int ri = r / (p8 / 8), dn0 = (r - ri * (p8 / 8)) * 8;
+  const int m = lo, rw = pk_rw(T, m);
+  const int r = u - pk_seg(T, m) / 24;  // chunk within the segment's 4 rows
+  const int row = r / (rw / 8), j0 = (r - row * (rw / 8)) * 8;
+  const int ri = row >> 1, h = row & 1;
   const float* src = sp + mem * sp_stride + (long)(f0 + field) * S::NRSP + 2 * w0_of(T, m) + ri;
-  BF8 h, md, l;
+  BF8 hi8, md8, lo8;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int dn = dn0 + j;  // n - m
+  for (int jj = 0; jj < 8; ++jj) {
+    const int dn = 2 * (j0 + jj) + h;  // n - m
     const float x = (m + dn <= T) ? src[2 * dn] : 0.f;
-    split3(x, h.s[j], md.s[j], l.s[j]);
+    split3(x, hi8.s[jj], md8.s[jj], lo8.s[jj]);
   }
-  uint16_t* dst = out + fm * pk_field(T) + pk_seg(T, m) + ri * p8 + dn0;
-  *(uint4*)dst = h.q;
-  *(uint4*)(dst + 2 * p8) = md.q;
-  *(uint4*)(dst + 4 * p8) = l.q;
+  uint16_t* dst = out + fm * pk_field(T) + pk_seg(T, m) + row * rw + j0;
+  *(uint4*)dst = hi8.q;
+  *(uint4*)(dst + 4 * rw) = md8.q;
+  *(uint4*)(dst + 8 * rw) = lo8.q;
 }
 
 template <int T>
 static int pack_t(const float* sp, long sps, int f0, int nf, int nmem, uint16_t* out,
                   hipStream_t s) {
-  const long total = (long)nmem * nf * (pk_c8(T + 1) / 4);
+  const long total = (long)nmem * nf * (pk_c16(T + 1) / 2);
   if (total <= 0) return 0;
   k_pack_sp<T><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(sp, sps, f0, nf, nmem, out);
   return hipGetLastError() == hipSuccess ? 0 : -3;
