@@ -341,8 +341,9 @@ __device__ __forceinline__ void fft_pass(C2<F>* buf, const C2<F>* tw, int nf,
       });
     }
   }
-  // in place: every read of the pass ends before the first write
-  if (!FIRST && !LAST) __syncthreads();
+  // in place (the last pass's store may write the same rows too): every
+  // read of the pass ends before the first write
+  if (!FIRST) __syncthreads();
 #pragma unroll
   for (int t = 0; t < BPT; ++t) {
     const int b = tid + t * NT;
