@@ -398,27 +398,29 @@ __device__ __forceinline__ void fft_run(C2<F>* buf, const C2<F>* tw, int nf, Loa
   }
 }
 
-// Full FFT of NFM sequences already in `buf` (rows of N + N/32 complex),
-// in place; with OUT the last pass writes through `store` instead.  Inputs
-// ZLO..ZHI of every row are zero (and must be zero in `buf`).
+// Full FFT of nf (<= NFM) sequences already in `buf` (rows of N + N/32
+// complex), in place; with OUT the last pass writes through `store`
+// instead.  Inputs ZLO..ZHI of every row are zero (and must be zero in
+// `buf`).
 template <int N, int S, typename F, int NT, int NFM, int ZLO, int ZHI, bool OUT = true,
           typename Store>
-__device__ __forceinline__ void fft_run_lds(C2<F>* buf, const C2<F>* tw, Store& store) {
+__device__ __forceinline__ void fft_run_lds(C2<F>* buf, const C2<F>* tw, Store& store,
+                                            int nf = NFM) {
   using P = FftPlan<N>;
   constexpr int R0 = P::R[0], R1 = P::R[1], R2 = P::R[2], R3 = P::R[3];
   auto none = [](int, int) { return C2<F>{F(0), F(0)}; };
   if constexpr (P::NP == 2) {
-    fft_pass<N, R0, 1, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
-    fft_pass<N, R1, R0, S, false, OUT, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
+    fft_pass<N, R0, 1, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, nf, none, store);
+    fft_pass<N, R1, R0, S, false, OUT, F, NT, NFM, ZLO, ZHI>(buf, tw, nf, none, store);
   } else if constexpr (P::NP == 3) {
-    fft_pass<N, R0, 1, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
-    fft_pass<N, R1, R0, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
-    fft_pass<N, R2, R0 * R1, S, false, OUT, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
+    fft_pass<N, R0, 1, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, nf, none, store);
+    fft_pass<N, R1, R0, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, nf, none, store);
+    fft_pass<N, R2, R0 * R1, S, false, OUT, F, NT, NFM, ZLO, ZHI>(buf, tw, nf, none, store);
   } else {
-    fft_pass<N, R0, 1, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
-    fft_pass<N, R1, R0, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
-    fft_pass<N, R2, R0 * R1, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none, store);
-    fft_pass<N, R3, R0 * R1 * R2, S, false, OUT, F, NT, NFM, ZLO, ZHI>(buf, tw, NFM, none,
+    fft_pass<N, R0, 1, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, nf, none, store);
+    fft_pass<N, R1, R0, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, nf, none, store);
+    fft_pass<N, R2, R0 * R1, S, false, false, F, NT, NFM, ZLO, ZHI>(buf, tw, nf, none, store);
+    fft_pass<N, R3, R0 * R1 * R2, S, false, OUT, F, NT, NFM, ZLO, ZHI>(buf, tw, nf, none,
                                                                       store);
   }
 }
