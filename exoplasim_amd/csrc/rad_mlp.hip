@@ -10,7 +10,7 @@
 //   k_fwd     Z = X W^T + b; H = gelu(Z) (bf16) or Y = Z (fp32, last layer)
 //   k_bwd_dx  dH = dZ W; dZprev = dH * gelu'(Zprev) (bf16 + fp32 copies)
 //   k_bwd_dw  split-K partial sums of dW = dZ^T X over the batch
-//   k_sum_dw  sum of the partials; k_bias_grad db = sum_m dZ
+//   k_sum_dw  sum of the partials; k_bias_grad partials of db = sum_m dZ
 //   k_cast_w  FP32 master weights -> W, Wt bf16 (zero padded)
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -220,18 +220,25 @@ __global__ void k_sum_dw(const float* part, int nsplit, long nk, float* dW) {
   dW[i] = s;
 }
 
-// db[n] = sum_m dZ[m][n] (fp32 input), one block per 64 columns
-__global__ __launch_bounds__(256) void k_bias_grad(const float* dZ, int M, int N, float* db) {
+// partial bias gradients: out[s][n] = sum of dZ[m][n] (fp32 input) over the
+// rows m of chunk s; one block per (64 columns, chunk), summed in a fixed
+// order by k_sum_dw
+__global__ __launch_bounds__(256) void k_bias_grad(const float* dZ, int M, int N, int mchunk,
+                                                   float* out) {
   __shared__ float red[4][64];
-  const int n = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int ntn = N / 64 > 0 ? N / 64 : 1;
+  const int tn = blockIdx.x % ntn, sp = blockIdx.x / ntn;
+  const int n = tn * 64 + (threadIdx.x & 63);
   const int part = threadIdx.x >> 6;
+  const int m1 = min(M, (sp + 1) * mchunk);
   float s = 0.f;
   if (n < N)
-    for (int m = part; m < M; m += 4) s += dZ[(long)m * N + n];
+    for (int m = sp * mchunk + part; m < m1; m += 4) s += dZ[(long)m * N + n];
   red[part][threadIdx.x & 63] = s;
   __syncthreads();
-  if (part == 0 && n < N) db[n] = red[0][threadIdx.x] + red[1][threadIdx.x] +
-                                  red[2][threadIdx.x] + red[3][threadIdx.x];
+  if (part == 0 && n < N)
+    out[(long)sp * N + n] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] +
+                            red[3][threadIdx.x];
 }
 
 // FP32 master W[nout][nin] -> padded bf16 W[N][K] and Wt[K][N]
@@ -430,7 +437,10 @@ extern "C" int exo_mlp_backward_dw(const void* dZ, const void* X, const float* d
   k_bwd_dw<<<ntn * ntk * ns, 256, 0, s>>>((const uint16_t*)dZ, (const uint16_t*)X, M, N, K,
                                            mchunk, part);
   k_sum_dw<<<nb((long)N * K, 256), 256, 0, s>>>(part, ns, (long)N * K, dW);
-  k_bias_grad<<<ntn, 256, 0, s>>>(dZ32, M, N, db);
+  // the bias partials reuse `part` (ns x N <= ns x N x K floats) once the
+  // weight partials are summed
+  k_bias_grad<<<ntn * ns, 256, 0, s>>>(dZ32, M, N, mchunk, part);
+  k_sum_dw<<<nb(N, 256), 256, 0, s>>>(part, ns, N, db);
   return hipGetLastError() == hipSuccess ? EXO_RAD_OK : EXO_RAD_ERR_HIP;
 }
 
