@@ -234,8 +234,11 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     auto fetch = [&](int m, int kb, BFrag& br, uint4* a0, uint4* a1) {
       if constexpr (MODE == MODE_BF16X3) {
         const int rw = pk_rw(T, m);
+        // lanes whose j lies past the row would read n > NTRU, where the
+        // table operand is zero: they load nothing
+        const bool inrow = 16 * kb + jq < rw;
         const int bso = (pk_seg(T, m) + 16 * kb) * 2;
-        const int bv = bvoff + brow * rw * 2;
+        const int bv = inrow ? bvoff + brow * rw * 2 : BUF_NONE;
 #pragma unroll
         for (int p = 0; p < 3; ++p) br.p[p].q = buffer_load_x4(rsB, bv, bso + p * rw * 8);
         if (m == 0 && kb == 0 && isvor && q == 2) {
@@ -248,7 +251,6 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
           split3(x - (float)pv, br.p[0].s[0], br.p[1].s[0], br.p[2].s[0]);
         }
         const int aso = (tab_seg(T, m) + 16 * kb) * 2;
-        const bool inrow = 16 * kb + jq < rw;
         const int av = (c * rw + jq) * 2;
         const int av0 = (amine0 && inrow) ? av : BUF_NONE;
 #pragma unroll
