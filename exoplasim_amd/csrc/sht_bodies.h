@@ -231,12 +231,18 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
       float f[8];
       BF8 p[3];
     };
-    auto fetch = [&](int m, int kb, BFrag& br, uint4* a0, uint4* a1) {
+    // Every step fetches, also past the last item (m > T: nothing is loaded,
+    // at the offsets of m = T), so that each step issues the same loads and
+    // the wait for an item's operands counts only the loads issued before
+    // them.
+    auto fetch = [&](int mi, int kb, BFrag& br, uint4* a0, uint4* a1) {
       if constexpr (MODE == MODE_BF16X3) {
+        const bool live = mi <= T;
+        const int m = live ? mi : T;
         const int rw = pk_rw(T, m);
         // lanes whose j lies past the row would read n > NTRU, where the
         // table operand is zero: they load nothing
-        const bool inrow = 16 * kb + jq < rw;
+        const bool inrow = live && 16 * kb + jq < rw;
         const int bso = (pk_seg(T, m) + 16 * kb) * 2;
         const int bv = inrow ? bvoff + brow * rw * 2 : BUF_NONE;
 #pragma unroll
@@ -261,7 +267,10 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
           for (int p = 0; p < 3; ++p) a1[p] = buffer_load_x4(rsA1, av1, aso + p * rw * 32);
         }
       } else {
-        const int w0 = w0_of(T, m);
+        // m > T: n > T loads nothing, and the table offset lies past the
+        // table's buffer
+        const int m = mi;
+        const int w0 = w0_of(T, m <= T ? m : T);
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
           const int n = m + kb * 32 + j * 4 + q;
@@ -324,7 +333,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     // item `ahead` (two items ahead) into `into`; at the last n block of an
     // m, combine the terms, fold N/S and store the Fourier coefficients
     auto step = [&](Item cur, Frag& use, Frag& into, Item ahead) {
-      if (ahead.m <= T) fetch(ahead.m, ahead.kb, into.b, into.a0, into.a1);
+      fetch(ahead.m, ahead.kb, into.b, into.a0, into.a1);
       mul(use.b, use.a0, acc1, acc1b);
       if (NTERM == 2) mul(use.b, use.a1, acc2, acc2b);
       if (next_item(cur).kb == 0) {
@@ -382,8 +391,8 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
       Frag f0, f1, f2;
       Item i0 = {wv, 0};
       Item i1 = next_item(i0), i2 = next_item(i1);
-      if (i0.m <= T) fetch(i0.m, i0.kb, f0.b, f0.a0, f0.a1);
-      if (i1.m <= T) fetch(i1.m, i1.kb, f1.b, f1.a0, f1.a1);
+      fetch(i0.m, i0.kb, f0.b, f0.a0, f0.a1);
+      fetch(i1.m, i1.kb, f1.b, f1.a0, f1.a1);
       while (i0.m <= T) {
         step(i0, f0, f2, i2);
         i0 = i1;
@@ -406,7 +415,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
       Frag f0, f1;
       Item i0 = {wv, 0};
       Item i1 = next_item(i0);
-      if (i0.m <= T) fetch(i0.m, i0.kb, f0.b, f0.a0, f0.a1);
+      fetch(i0.m, i0.kb, f0.b, f0.a0, f0.a1);
       while (i0.m <= T) {
         step(i0, f0, f1, i1);
         i0 = i1;
