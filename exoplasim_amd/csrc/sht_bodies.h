@@ -647,7 +647,7 @@ __device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* 
     R acc[NCOL];
 #pragma unroll
     for (int c = 0; c < NCOL; ++c) acc[c] = R(0);
-#pragma unroll 4
+#pragma unroll 8
     for (int pl = 0; pl < PB; ++pl) {
       const R p = buffer_load<R>(trs, w * (int)sizeof(R), pl * S::NCSP * (int)sizeof(R));
 #pragma unroll
