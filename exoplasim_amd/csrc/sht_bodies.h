@@ -324,7 +324,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
       int m, kb;
     };
     auto next_item = [&](Item it) -> Item {
-      const int kbn = (NM - it.m + 31) / 32;
+      const int kbn = (int)((unsigned)(NM - it.m + 31) >> 5);  // m <= T + NW
       return (it.kb + 1 < kbn) ? Item{it.m, it.kb + 1} : Item{it.m + NW, 0};
     };
     const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};

@@ -64,10 +64,15 @@ __device__ __forceinline__ int m_of_w(int ntru, int w) {
 // k = 16 + j holds n - m = 32 kb + 2 j + 1 (j = 0..15).  Both operands are
 // stored as rows of one parity h: element j of a row is n - m = 2 j + h,
 // pk_rw(m) = 8 ceil((NTRU + 1 - m) / 16) values per row, zero for n > NTRU.
-// pk_c16(x) = sum_{L=1..x} 8 ceil(L/16) sums the row lengths.
-__host__ __device__ constexpr int pk_rw(int ntru, int m) { return 8 * ((ntru + 16 - m) / 16); }
+// pk_c16(x) = sum_{L=1..x} 8 ceil(L/16) sums the row lengths.  Both take
+// m <= NTRU + 1 and x >= 0 and compute in unsigned shifts (these offsets are
+// evaluated for every work item of the inverse, in scalar registers).
+__host__ __device__ constexpr int pk_rw(int ntru, int m) {
+  return 8 * (int)((unsigned)(ntru + 16 - m) >> 4);
+}
 __host__ __device__ constexpr int pk_c16(int x) {
-  return 64 * (x / 16) * (x / 16 + 1) + 8 * (x % 16) * (x / 16 + 1);
+  const unsigned a = (unsigned)x >> 4, b = (unsigned)x & 15u;
+  return (int)(64u * a * (a + 1u) + 8u * b * (a + 1u));
 }
 // spectral operand: per field, for each m, [plane hi/mid/lo][re/im][h][pk_rw]
 __host__ __device__ constexpr int pk_field(int ntru) { return 12 * pk_c16(ntru + 1); }
