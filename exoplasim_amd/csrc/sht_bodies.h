@@ -166,15 +166,15 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     }
   };
 
-  // ---- phase 1: Legendre sums into fcl ----
+  // ---- phase 1: Legendre sums, into the FFT rows (fused) or fcl ----
   if constexpr (MODE == MODE_BF16X3 || MODE == MODE_F32MFMA) {
     // GEMM per m: C[row = 2*pair + class][col] = sum_n A[row][n] B[n][col];
-    // wave wv takes m = wv, wv+4, ...; each (m, 32-wide n block) is one
-    // work item, and the operands of the next item are fetched before the
-    // current one is multiplied.
-    // the wave index is read into a scalar register, so that the item loop
+    // wave wv of NW takes m = wv, wv + NW, ...; each (m, 32-wide n block) is
+    // one work item, and the operands of the items ahead are fetched before
+    // the current one is multiplied.
+    // The wave index is read into a scalar register, so that the item loop
     // (m, kb), its branches and the operand offsets are wave-uniform scalar
-    // work rather than per-lane vector arithmetic
+    // work rather than per-lane vector arithmetic.
     const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int q = lane >> 4, c = lane & 15;
     // column c of B: SCALAR field g*FB + c/2; UV level g*FB/2 + c/4 of
