@@ -571,8 +571,11 @@ __device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* 
   const int ngroups = (a.nfield + FBF - 1) / FBF;
   const int pb = blk % NPB;
   const int g = (blk / NPB) % ngroups;
-  const int mem = blk / (NPB * ngroups);
-  if (mem >= a.nmem) return;
+  const int mem0 = blk / (NPB * ngroups);
+  if (mem0 >= a.nmem) return;
+  // members in reverse order: in a round trip the inverse has just written
+  // the grids, and its last members are the likeliest to be still cached
+  const int mem = a.nmem - 1 - mem0;
   const R* inm = (const R*)a.in + mem * a.in_stride;
 
   if (FUSED) {
