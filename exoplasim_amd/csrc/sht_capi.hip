@@ -456,12 +456,14 @@ extern "C" int exo_sht_autotune(exo_sht_plan_t p, int mode, int nmem, int nlev, 
   int rc = EXO_OK;
   float best[3] = {1e30f, 1e30f, 1e30f};
   int pick[3] = {0, 0, 0};
+  // each variant of each kernel: 2 untimed calls, then the average of 8
+  // back-to-back calls (steady state, no host round trip between them)
+  constexpr int WARM = 2, REPS = 8;
   for (int v = 0; v < 3 && rc == EXO_OK; ++v) {
     p->inv_var[mode] = p->uv_var[mode] = p->fwd_var[mode] = v;
     for (int k = 0; k < 3 && rc == EXO_OK; ++k) {
-      float ms_min = 1e30f;
-      for (int rep = 0; rep < 4 && rc == EXO_OK; ++rep) {
-        (void)hipEventRecord(e0, s);
+      for (int rep = 0; rep < WARM + REPS && rc == EXO_OK; ++rep) {
+        if (rep == WARM) (void)hipEventRecord(e0, s);
         if (k == 0)
           rc = exo_sht_inverse_all(p, mode, nmem, nlev, sp, gp, s);
         else if (k == 1)
@@ -469,14 +471,14 @@ extern "C" int exo_sht_autotune(exo_sht_plan_t p, int mode, int nmem, int nlev, 
                                   nfl + 1 + nlev, nlev, s);
         else
           rc = exo_sht_direct(p, mode, nmem, gp, gps, 0, so, sps, 0, nfl, s);
-        (void)hipEventRecord(e1, s);
-        (void)hipEventSynchronize(e1);
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (rep > 0 && ms < ms_min) ms_min = ms;  // first run warms up
       }
-      if (ms_min < best[k]) {
-        best[k] = ms_min;
+      (void)hipEventRecord(e1, s);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      ms /= REPS;
+      if (rc == EXO_OK && ms < best[k]) {
+        best[k] = ms;
         pick[k] = v;
       }
     }

@@ -167,8 +167,10 @@ int exo_sht_inverse_all(exo_sht_plan_t plan, int mode, int nmem, int nlev, const
  * vars[2] the direct kernel; 0 = default tiles, 1 = half the field-levels
  * per block (half the LDS, twice the blocks in flight), 2 = default tiles
  * on 512-thread blocks (twice the waves per block).  exo_sht_autotune
- * times both on a round trip of nmem members x nlev levels, keeps the
- * fastest of each for this mode and returns them in vars[3] (may be NULL).
+ * times each variant (two untimed calls, then the average of eight
+ * back-to-back calls) on a round trip of nmem members x nlev levels, keeps
+ * the fastest of each for this mode and returns them in vars[3] (may be
+ * NULL).
  * Calls on one plan share its scratch buffers: calls that overlap on
  * different streams race. */
 int exo_sht_set_tiles(exo_sht_plan_t plan, int mode, int inv_var, int uv_var, int fwd_var);
