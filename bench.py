@@ -364,7 +364,8 @@ def main():
         }
         kt = {}
         for name, (fn, nbytes) in legs.items():
-            tk = time_loop(fn, args.steps, 2) / args.steps
+            # median of three timed loops of K calls each
+            tk = sorted(time_loop(fn, args.steps, 2) for _ in range(3))[1] / args.steps
             kt[name] = {"ms": tk * 1e3, "bytes": nbytes, "GBs": nbytes / tk / 1e9}
         plan.roundtrip(mode, sp_in, gp, sp_out, NLEV)
         torch.cuda.synchronize()
