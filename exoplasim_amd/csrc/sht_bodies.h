@@ -578,6 +578,36 @@ __device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* 
   const int mem = a.nmem - 1 - mem0;
   const R* inm = (const R*)a.in + mem * a.in_stride;
 
+  // Rows pl x field of the tile: row gi holds pair pl = gi / FBF, field
+  // g FBF + gi % FBF; z_k = g_N(k) + i g_S(k) of the northern and southern
+  // latitudes of that pair (zero for fields past nfield).
+  auto grid_z = [&](int gi, int k) -> C2<R> {
+    const int pl = gi / FBF, fl = gi % FBF;
+    const int field = g * FBF + fl;
+    if (field >= a.nfield) return C2<R>{R(0), R(0)};
+    const int lat = pb * PB + pl;
+    const R* src = inm + (long)(a.src0 + field) * NLAT * N;
+    return C2<R>{src[(long)lat * N + k], src[(long)(NLAT - 1 - lat) * N + k]};
+  };
+
+  // When the first FFT pass of every batch fits one butterfly per thread,
+  // each thread requests its butterfly's inputs here, so that the grid
+  // rows of all batches are in flight together
+  constexpr int R0 = FftPlan<N>::R[0], NB0 = N / R0;
+  constexpr int NBATCH = (PB * FBF + NF - 1) / NF;
+  constexpr bool HELD = FUSED && NBATCH * NF * NB0 <= NT;
+  C2<R> held[R0];
+  const int hb = tid % (NF * NB0);               // butterfly within its batch
+  const int hbatch = tid / (NF * NB0);           // batch of that butterfly
+  const int hrow = hbatch * NF + hb / NB0;       // tile row
+  if constexpr (HELD) {
+    if (hrow < PB * FBF) {
+      const int j = hb % NB0;
+#pragma unroll
+      for (int r = 0; r < R0; ++r) held[r] = grid_z(hrow, j + r * NB0);
+    }
+  }
+
   if (FUSED) {
     const C2<R>* tw = (const C2<R>*)twp;
     for (int k = tid; k < N; k += NT) tws[k] = C2<R>{tw[k].x, -tw[k].y};
@@ -586,17 +616,33 @@ __device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* 
 
   auto fidx = [](int m, int cls, int pl, int col) { return (m * 2 + cls) * L::RS + pl * NCOL + col; };
 
-  if constexpr (FUSED) {
+  if constexpr (HELD) {
+    for (int bt = 0; bt < NBATCH; ++bt) {
+      const int f0 = bt * NF;
+      const int nf = min(NF, PB * FBF - f0);
+      auto store = [&](int f, int k, C2<R> v) { fbuf[bix(f * N + k)] = v; };
+      fft_pass0_held<N, -1, R>(fbuf, hbatch == bt && hrow < PB * FBF, hb, held);
+      fft_run_after0<N, -1, R, NT, NF>(fbuf, tws, store, nf);
+      // separate the two real rows, scale by 1/N, fold into E/O
+      const R sc = R(0.5) / R(N);
+      for (int idx = tid; idx < nf * NM; idx += NT) {
+        const int f = idx % nf, k = idx / nf;
+        const C2<R> zk = fbuf[bix(f * N + k)];
+        const C2<R> zn = fbuf[bix(f * N + ((N - k) % N))];
+        const R cnx = (zk.x + zn.x) * sc, cny = (zk.y - zn.y) * sc;
+        const R csx = (zk.y + zn.y) * sc, csy = (zn.x - zk.x) * sc;
+        const int gi = f0 + f, pl = gi / FBF, fl = gi % FBF;
+        F[fidx(k, 0, pl, 2 * fl)] = cnx + csx;
+        F[fidx(k, 0, pl, 2 * fl + 1)] = cny + csy;
+        F[fidx(k, 1, pl, 2 * fl)] = cnx - csx;
+        F[fidx(k, 1, pl, 2 * fl + 1)] = cny - csy;
+      }
+      __syncthreads();
+    }
+  } else if constexpr (FUSED) {
     for (int f0 = 0; f0 < PB * FBF; f0 += NF) {
       const int nf = min(NF, PB * FBF - f0);
-      auto load = [&](int f, int k) -> C2<R> {
-        const int gi = f0 + f, pl = gi / FBF, fl = gi % FBF;
-        const int field = g * FBF + fl;
-        if (field >= a.nfield) return C2<R>{R(0), R(0)};
-        const int lat = pb * PB + pl;
-        const R* src = inm + (long)(a.src0 + field) * NLAT * N;
-        return C2<R>{src[(long)lat * N + k], src[(long)(NLAT - 1 - lat) * N + k]};
-      };
+      auto load = [&](int f, int k) -> C2<R> { return grid_z(f0 + f, k); };
       auto store = [&](int f, int k, C2<R> v) { fbuf[bix(f * N + k)] = v; };
       fft_run<N, -1, R, NT>(fbuf, tws, nf, load, store);
       // separate the two real rows, scale by 1/N, fold into E/O

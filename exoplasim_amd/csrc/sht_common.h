@@ -403,6 +403,58 @@ __device__ __forceinline__ void fft_run(C2<F>* buf, const C2<F>* tw, int nf, Loa
   }
 }
 
+// The first pass (radix R = FftPlan<N>::R[0], stride 1) of a Stockham FFT
+// whose inputs the calling thread already holds: butterfly b = f NB + j
+// (NB = N / R) takes inputs k = j + r NB, r < R, from v.  Threads with
+// active == false hold no butterfly of this batch.  The outputs go to the
+// rows of `buf` (N + N/32 complex each); all threads of the block must call
+// it.
+template <int N, int S, typename F>
+__device__ __forceinline__ void fft_pass0_held(C2<F>* buf, bool active, int b,
+                                               const C2<F> (&v)[FftPlan<N>::R[0]]) {
+  constexpr int R = FftPlan<N>::R[0];
+  constexpr int NB = N / R;
+  constexpr int NP = N + N / 32;
+  if (active) {
+    C2<F> w[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) w[r] = v[r];
+    Dft<R, S, F>::run(w);
+    const int f = b / NB, j = b - f * NB;
+    const int idxd = j * R;
+    const int wb = f * NP + idxd + (idxd >> 5);
+    static_for<0, R>([&](auto rc) {
+      constexpr int r = decltype(rc)::value;
+      constexpr int D = pad_delta<NB, 1, R, r, true>();
+      if constexpr (D >= 0)
+        buf[wb + r + D] = w[r];
+      else
+        buf[f * NP + idxd + r + ((idxd + r) >> 5)] = w[r];
+    });
+  }
+  __syncthreads();
+}
+
+// The passes after the first of the FFTs of nf (<= NFM) sequences whose
+// first pass is in `buf`; the last pass writes through `store`.
+template <int N, int S, typename F, int NT, int NFM, typename Store>
+__device__ __forceinline__ void fft_run_after0(C2<F>* buf, const C2<F>* tw, Store& store,
+                                               int nf) {
+  using P = FftPlan<N>;
+  constexpr int R0 = P::R[0], R1 = P::R[1], R2 = P::R[2], R3 = P::R[3];
+  auto none = [](int, int) { return C2<F>{F(0), F(0)}; };
+  if constexpr (P::NP == 2) {
+    fft_pass<N, R1, R0, S, false, true, F, NT, NFM, 1, 0>(buf, tw, nf, none, store);
+  } else if constexpr (P::NP == 3) {
+    fft_pass<N, R1, R0, S, false, false, F, NT, NFM, 1, 0>(buf, tw, nf, none, store);
+    fft_pass<N, R2, R0 * R1, S, false, true, F, NT, NFM, 1, 0>(buf, tw, nf, none, store);
+  } else {
+    fft_pass<N, R1, R0, S, false, false, F, NT, NFM, 1, 0>(buf, tw, nf, none, store);
+    fft_pass<N, R2, R0 * R1, S, false, false, F, NT, NFM, 1, 0>(buf, tw, nf, none, store);
+    fft_pass<N, R3, R0 * R1 * R2, S, false, true, F, NT, NFM, 1, 0>(buf, tw, nf, none, store);
+  }
+}
+
 // Full FFT of nf (<= NFM) sequences already in `buf` (rows of N + N/32
 // complex), in place; with OUT the last pass writes through `store`
 // instead.  Inputs ZLO..ZHI of every row are zero (and must be zero in
