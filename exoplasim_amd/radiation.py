@@ -3,8 +3,8 @@
 * ``LwrColumn``: the reference's long-wave column (src/radmod.f90:2122-2456,
   subroutine lwr) with its heating rate (src/radmod.f90:1100-1108) as a HIP
   kernel, one thread per column (include/exo_rad.h: exo_lwr).  It is the
-  emulator's training-data generator, its fallback and the judge of its
-  heating-rate error.
+  emulator's training-data generator, its fallback and the reference for
+  its heating-rate error.
 * ``LwEmulator``: a 43 -> 256 -> 256 -> 256 -> 21 GELU MLP whose forward and
   backward passes run on this package's own BF16 MFMA kernels
   (exo_mlp_forward / exo_mlp_backward_dx / exo_mlp_backward_dw), wrapped as a

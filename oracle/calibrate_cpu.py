@@ -2,7 +2,7 @@
 legmod/fftmod (oracle/_ref/libref_t21.so, T21 L10) on one host core, on the
 same seeded member, after checking that both give the same arrays.
 TEST INFRASTRUCTURE: the ratio calibrates the CPU baseline that bench.py
-measures with the oracle on the GPU box's host cores.
+measures with the oracle on the host cores of the benchmark machine.
 
     make -C oracle ref && python oracle/calibrate_cpu.py
 """

@@ -1,7 +1,7 @@
 /*
  * exo_oracle.h -- FP64 CPU restatement of PlaSim's spectral-transform core.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * TEST INFRASTRUCTURE ONLY.  This library is the test oracle: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
  * product (exoplasim_amd) never links or calls it.
  *

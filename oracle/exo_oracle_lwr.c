@@ -3,8 +3,8 @@
  * of the reference (src/radmod.f90:2122-2456, subroutine lwr) and of its
  * heating-rate conversion (src/radmod.f90:1100-1108).
  *
- * TEST INFRASTRUCTURE ONLY (the checker of the GPU column kernel and the
- * judge of the learned emulator); the product never links it.
+ * TEST INFRASTRUCTURE ONLY (the CPU column that the GPU column kernel and the
+ * learned emulator are tested against); the product never links it.
  *
  * One call transforms ncol independent columns.  Arrays are [col][lev]
  * (one column contiguous); level 1 of the reference is index 0 (top) and
