@@ -189,7 +189,9 @@ def bench_t127(args, ora):
     es = sp_in.element_size()
     nbytes = nmem * (nfl + 1 + 2 * nlev + nfl) * (plan.nrsp + plan.nlat * plan.nlon) * es
     t_qi = time_loop(lambda: plan.inverse(mode, sp_in, gp, nfl, 0, 0, "qi"), steps, 2) / steps
-    fb = 4 if tiles[0] == 0 else 2
+    # fields per block of the qi launch: 4 at T127, halved by tile variant 1
+    # only (variant 2 keeps the fields and doubles the threads)
+    fb = 2 if tiles[0] == 1 else 4
     fl_qi = _mfma_flops_inverse(ntru, nfl, nmem, fb, 1)
     torch.cuda.synchronize()
     plan.roundtrip(mode, sp_in, gp, sp_out, nlev)
