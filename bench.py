@@ -10,6 +10,13 @@ The headline value is the split-BF16 (three-way split, FP32 accumulate, MFMA)
 path; the plain FP64 GPU path is timed beside it, and both are checked
 against the FP64 oracle on two members.
 
+The oracle (oracle/, test infrastructure) appears here in two roles only:
+as the checker of the outputs the GPU already produced (the error fields,
+computed on the host after every timed loop has ended), and as the timed
+CPU baseline (cpu_baseline, and the plain CPU column of the emulator leg).
+No timed GPU region calls it, and the transforms fail without the HIP
+library instead of falling back to it.
+
 Prints one JSON line (rank 0, N=1).
 """
 from __future__ import annotations
