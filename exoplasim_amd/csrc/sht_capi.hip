@@ -373,6 +373,7 @@ extern "C" int exo_sht_set_tiles(exo_sht_plan_t p, int mode, int inv_var, int uv
   if (!p || !mode_ok(mode) || inv_var < 0 || inv_var > 2 || uv_var < 0 || uv_var > 2 ||
       fwd_var < 0 || fwd_var > 2)
     return EXO_ERR_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
   p->inv_var[mode] = inv_var;
   p->uv_var[mode] = uv_var;
   p->fwd_var[mode] = fwd_var;
@@ -381,6 +382,7 @@ extern "C" int exo_sht_set_tiles(exo_sht_plan_t p, int mode, int inv_var, int uv
 
 extern "C" int exo_sht_get_tiles(exo_sht_plan_t p, int mode, int* vars) {
   if (!p || !mode_ok(mode) || !vars) return EXO_ERR_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
   vars[0] = p->inv_var[mode];
   vars[1] = p->uv_var[mode];
   vars[2] = p->fwd_var[mode];
@@ -435,24 +437,39 @@ extern "C" int exo_sht_inverse_all(exo_sht_plan_t p, int mode, int nmem, int nle
   return inverse_all_impl(p, mode, nmem, nlev, sp_in, gp, (hipStream_t)stream);
 }
 
+// device buffers and events of one autotune run, released on every return
+struct TuneScratch {
+  void *sp = nullptr, *gp = nullptr, *so = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ~TuneScratch() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (sp) (void)hipFree(sp);
+    if (gp) (void)hipFree(gp);
+    if (so) (void)hipFree(so);
+  }
+};
+
 // time each tile variant of the three fused kernels on a round trip of
-// nmem members x nlev levels and keep the fastest
+// nmem members x nlev levels and keep the fastest; the plan's lock is held
+// for the whole run, so no other call sees an intermediate tile choice
 extern "C" int exo_sht_autotune(exo_sht_plan_t p, int mode, int nmem, int nlev, void* stream,
                                 int* vars) {
   if (!p || !mode_ok(mode) || nmem < 1 || nlev < 1) return EXO_ERR_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
   const Tables& t = p->tb;
   hipStream_t s = (hipStream_t)stream;
   const int nfl = 3 * nlev + 1, ngp = nfl + 1 + 2 * nlev;
   const long sps = (long)nfl * 2 * t.ncsp, gps = (long)ngp * t.nlat * t.nlon;
   const size_t es = (mode == MODE_F64) ? 8 : 4;
-  void *sp = nullptr, *gp = nullptr, *so = nullptr;
-  HIPCK(hipMalloc(&sp, es * sps * nmem));
-  HIPCK(hipMalloc(&gp, es * gps * nmem));
-  HIPCK(hipMalloc(&so, es * sps * nmem));
-  HIPCK(hipMemsetAsync(sp, 0x3f, es * sps * nmem, s));  // finite, nonzero data
-  hipEvent_t e0, e1;
-  HIPCK(hipEventCreate(&e0));
-  HIPCK(hipEventCreate(&e1));
+  TuneScratch b;
+  HIPCK(hipMalloc(&b.sp, es * sps * nmem));
+  HIPCK(hipMalloc(&b.gp, es * gps * nmem));
+  HIPCK(hipMalloc(&b.so, es * sps * nmem));
+  HIPCK(hipMemsetAsync(b.sp, 0x3f, es * sps * nmem, s));  // finite, nonzero data
+  HIPCK(hipEventCreate(&b.e0));
+  HIPCK(hipEventCreate(&b.e1));
+  const int keep[3] = {p->inv_var[mode], p->uv_var[mode], p->fwd_var[mode]};
   int rc = EXO_OK;
   float best[3] = {1e30f, 1e30f, 1e30f};
   int pick[3] = {0, 0, 0};
@@ -463,37 +480,44 @@ extern "C" int exo_sht_autotune(exo_sht_plan_t p, int mode, int nmem, int nlev, 
     p->inv_var[mode] = p->uv_var[mode] = p->fwd_var[mode] = v;
     for (int k = 0; k < 3 && rc == EXO_OK; ++k) {
       for (int rep = 0; rep < WARM + REPS && rc == EXO_OK; ++rep) {
-        if (rep == WARM) (void)hipEventRecord(e0, s);
+        if (rep == WARM && hipEventRecord(b.e0, s) != hipSuccess) rc = EXO_ERR_HIP;
+        if (rc != EXO_OK) break;
         if (k == 0)
-          rc = exo_sht_inverse_all(p, mode, nmem, nlev, sp, gp, s);
+          rc = inverse_all_impl(p, mode, nmem, nlev, b.sp, b.gp, s);
         else if (k == 1)
-          rc = exo_sht_inverse_uv(p, mode, nmem, sp, sps, 0, nlev, gp, gps, nfl + 1,
-                                  nfl + 1 + nlev, nlev, s);
+          rc = inverse_uv_impl(p, mode, nmem, b.sp, sps, 0, nlev, b.gp, gps, nfl + 1,
+                               nfl + 1 + nlev, nlev, s, nullptr);
         else
-          rc = exo_sht_direct(p, mode, nmem, gp, gps, 0, so, sps, 0, nfl, s);
+          rc = direct_impl(p, mode, true, nmem, b.gp, gps, 0, b.so, sps, 0, nfl, s);
       }
-      (void)hipEventRecord(e1, s);
-      (void)hipEventSynchronize(e1);
+      if (rc != EXO_OK) break;
       float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (hipEventRecord(b.e1, s) != hipSuccess || hipEventSynchronize(b.e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, b.e0, b.e1) != hipSuccess) {
+        rc = EXO_ERR_HIP;
+        break;
+      }
       ms /= REPS;
-      if (rc == EXO_OK && ms < best[k]) {
+      if (ms < best[k]) {
         best[k] = ms;
         pick[k] = v;
       }
     }
+  }
+  if (rc != EXO_OK) {
+    // a failed run leaves the tile choice as it was
+    p->inv_var[mode] = keep[0];
+    p->uv_var[mode] = keep[1];
+    p->fwd_var[mode] = keep[2];
+    (void)hipStreamSynchronize(s);
+    return rc;
   }
   p->inv_var[mode] = pick[0];
   p->uv_var[mode] = pick[1];
   p->fwd_var[mode] = pick[2];
   if (vars)
     for (int k = 0; k < 3; ++k) vars[k] = pick[k];
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(sp);
-  (void)hipFree(gp);
-  (void)hipFree(so);
-  return rc;
+  return EXO_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -536,12 +560,21 @@ extern "C" int exo_legini_tables(double* tables) {
   return copy_tables(g_plan->tb, tables);
 }
 
-// stage host arrays through device memory, run, copy back (synchronous)
-template <typename R, typename Fn>
-static int staged(size_t in_elems, const R* in, size_t out_elems, R* out, Fn run) {
+// run fn on the current drop-in plan with g_mu held: every read of the plan
+// (its sizes and tables) happens under the lock, so a concurrent exo_legini
+// cannot replace the plan in between
+template <typename Fn>
+static int with_plan(Fn fn) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g_plan) return EXO_ERR_NOINIT;
-  exo_sht_plan_s* p = g_plan;
+  return fn(g_plan);
+}
+
+// stage host arrays through the plan's device buffer, run, copy back
+// (synchronous); the caller holds g_mu
+template <typename R, typename Fn>
+static int staged(exo_sht_plan_s* p, size_t in_elems, const R* in, size_t out_elems, R* out,
+                  Fn run) {
   const size_t need = sizeof(R) * (in_elems + out_elems);
   int rc = grow(&p->stage, &p->stage_bytes, need);
   if (rc) return rc;
@@ -558,24 +591,25 @@ static int staged(size_t in_elems, const R* in, size_t out_elems, R* out, Fn run
 template <typename R>
 static int sp2fc_impl(const R* sp, R* fc, int nfl, int table) {
   if (!sp || !fc || nfl < 0) return EXO_ERR_ARG;
-  if (!g_plan) return EXO_ERR_NOINIT;
-  const Tables& t = g_plan->tb;
-  const long nsp = 2L * t.ncsp, nhor = (long)t.nlat * t.nlon;
   const int mode = sizeof(R) == 8 ? MODE_F64 : MODE_F32;
-  return staged<R>(nsp * nfl, sp, nhor * nfl, fc, [&](exo_sht_plan_s* p, R* din, R* dout) {
-    InvArgs a{};
-    a.sp = din;
-    a.out = dout;
-    a.sp_stride = 0;
-    a.out_stride = 0;
-    a.nmem = 1;
-    a.kind = INV_SCALAR;
-    a.table = table;
-    a.src0 = 0;
-    a.dst0 = 0;
-    a.nfield = nfl;
-    a.plavor = p->tb.plavor;
-    return launch_inverse(p->tb, mode, false, a, 0);
+  return with_plan([&](exo_sht_plan_s* pl) {
+    const Tables& t = pl->tb;
+    const long nsp = 2L * t.ncsp, nhor = (long)t.nlat * t.nlon;
+    return staged<R>(pl, nsp * nfl, sp, nhor * nfl, fc, [&](exo_sht_plan_s* p, R* din, R* dout) {
+      InvArgs a{};
+      a.sp = din;
+      a.out = dout;
+      a.sp_stride = 0;
+      a.out_stride = 0;
+      a.nmem = 1;
+      a.kind = INV_SCALAR;
+      a.table = table;
+      a.src0 = 0;
+      a.dst0 = 0;
+      a.nfield = nfl;
+      a.plavor = p->tb.plavor;
+      return launch_inverse(p->tb, mode, false, a, 0);
+    });
   });
 }
 
@@ -595,12 +629,13 @@ extern "C" int exo_sp2fcdmu_f32(const float* sp, float* fc, int nfl) {
 template <typename R>
 static int fc2sp_impl(const R* fc, R* sp, int nfl) {
   if (!sp || !fc || nfl < 0) return EXO_ERR_ARG;
-  if (!g_plan) return EXO_ERR_NOINIT;
-  const Tables& t = g_plan->tb;
-  const long nsp = 2L * t.ncsp, nhor = (long)t.nlat * t.nlon;
   const int mode = sizeof(R) == 8 ? MODE_F64 : MODE_F32;
-  return staged<R>(nhor * nfl, fc, nsp * nfl, sp, [&](exo_sht_plan_s* p, R* din, R* dout) {
-    return direct_impl(p, mode, false, 1, din, 0, 0, dout, 0, 0, nfl, 0);
+  return with_plan([&](exo_sht_plan_s* pl) {
+    const Tables& t = pl->tb;
+    const long nsp = 2L * t.ncsp, nhor = (long)t.nlat * t.nlon;
+    return staged<R>(pl, nhor * nfl, fc, nsp * nfl, sp, [&](exo_sht_plan_s* p, R* din, R* dout) {
+      return direct_impl(p, mode, false, 1, din, 0, 0, dout, 0, 0, nfl, 0);
+    });
   });
 }
 
@@ -614,33 +649,34 @@ extern "C" int exo_fc2sp_f32(const float* fc, float* sp, int nfl) {
 template <typename R>
 static int dv2uv_impl(const R* pd, const R* pz, R* pu, R* pv) {
   if (!pd || !pz || !pu || !pv) return EXO_ERR_ARG;
-  if (!g_plan) return EXO_ERR_NOINIT;
-  const Tables& t = g_plan->tb;
-  const int nlev = g_plan->nlev;
-  const long nsp = 2L * t.ncsp, nhor = (long)t.nlat * t.nlon;
   const int mode = sizeof(R) == 8 ? MODE_F64 : MODE_F32;
-  std::vector<R> in(2 * nsp * nlev), out(2 * nhor * nlev);
-  memcpy(in.data(), pz, sizeof(R) * nsp * nlev);
-  memcpy(in.data() + nsp * nlev, pd, sizeof(R) * nsp * nlev);
-  int rc = staged<R>(in.size(), in.data(), out.size(), out.data(),
-                     [&](exo_sht_plan_s* p, R* din, R* dout) {
-                       InvArgs a{};
-                       a.sp = din;
-                       a.out = dout;
-                       a.nmem = 1;
-                       a.kind = INV_UV;
-                       a.vor0 = 0;
-                       a.div0 = nlev;
-                       a.u0 = 0;
-                       a.v0 = nlev;
-                       a.nlev = nlev;
-                       a.plavor = p->tb.plavor;
-                       return launch_inverse(p->tb, mode, false, a, 0);
-                     });
-  if (rc) return rc;
-  memcpy(pu, out.data(), sizeof(R) * nhor * nlev);
-  memcpy(pv, out.data() + nhor * nlev, sizeof(R) * nhor * nlev);
-  return EXO_OK;
+  return with_plan([&](exo_sht_plan_s* pl) {
+    const Tables& t = pl->tb;
+    const int nlev = pl->nlev;
+    const long nsp = 2L * t.ncsp, nhor = (long)t.nlat * t.nlon;
+    std::vector<R> in(2 * nsp * nlev), out(2 * nhor * nlev);
+    memcpy(in.data(), pz, sizeof(R) * nsp * nlev);
+    memcpy(in.data() + nsp * nlev, pd, sizeof(R) * nsp * nlev);
+    int rc = staged<R>(pl, in.size(), in.data(), out.size(), out.data(),
+                       [&](exo_sht_plan_s* p, R* din, R* dout) {
+                         InvArgs a{};
+                         a.sp = din;
+                         a.out = dout;
+                         a.nmem = 1;
+                         a.kind = INV_UV;
+                         a.vor0 = 0;
+                         a.div0 = nlev;
+                         a.u0 = 0;
+                         a.v0 = nlev;
+                         a.nlev = nlev;
+                         a.plavor = p->tb.plavor;
+                         return launch_inverse(p->tb, mode, false, a, 0);
+                       });
+    if (rc) return rc;
+    memcpy(pu, out.data(), sizeof(R) * nhor * nlev);
+    memcpy(pv, out.data() + nhor * nlev, sizeof(R) * nhor * nlev);
+    return EXO_OK;
+  });
 }
 
 extern "C" int exo_dv2uv_f64(const double* pd, const double* pz, double* pu, double* pv) {
@@ -653,13 +689,14 @@ extern "C" int exo_dv2uv_f32(const float* pd, const float* pz, float* pu, float*
 template <typename R>
 static int fft_impl(R* a, int n, int lot, int dir) {
   if (!a || lot < 0) return EXO_ERR_ARG;
-  if (!g_plan) return EXO_ERR_NOINIT;
-  if (n != g_plan->tb.nlon) return EXO_ERR_UNSUPPORTED;
   const size_t el = (size_t)n * lot;
   const int mode = sizeof(R) == 8 ? MODE_F64 : MODE_F32;
-  return staged<R>(el, a, el, a, [&](exo_sht_plan_s* p, R* din, R* dout) {
-    HIPCK(hipMemcpyAsync(dout, din, sizeof(R) * el, hipMemcpyDeviceToDevice, 0));
-    return launch_fft(p->tb, mode, dir, dout, n, lot, 0);
+  return with_plan([&](exo_sht_plan_s* pl) {
+    if (n != pl->tb.nlon) return EXO_ERR_UNSUPPORTED;
+    return staged<R>(pl, el, a, el, a, [&](exo_sht_plan_s* p, R* din, R* dout) {
+      HIPCK(hipMemcpyAsync(dout, din, sizeof(R) * el, hipMemcpyDeviceToDevice, 0));
+      return launch_fft(p->tb, mode, dir, dout, n, lot, 0);
+    });
   });
 }
 
@@ -695,26 +732,26 @@ static int dirx_one(exo_sht_plan_s* p, const R* const* din, int nin, R* dout,
 // stage nin Fourier inputs and nout spectral outputs (all nlev levels)
 template <typename R, typename Fn>
 static int dirx_staged(const R* const* in, int nin, R* const* out, int nout, Fn run) {
-  if (!g_plan) return EXO_ERR_NOINIT;
-  const Tables& t = g_plan->tb;
-  const long nhor = (long)t.nlat * t.nlon * g_plan->nlev;
-  const long nsp = 2L * t.ncsp * g_plan->nlev;
-  std::vector<R> hin(nhor * nin), hout(nsp * nout);
-  for (int i = 0; i < nin; ++i) {
+  for (int i = 0; i < nin; ++i)
     if (!in[i]) return EXO_ERR_ARG;
-    memcpy(hin.data() + i * nhor, in[i], sizeof(R) * nhor);
-  }
   for (int o = 0; o < nout; ++o)
     if (!out[o]) return EXO_ERR_ARG;
-  int rc = staged<R>(hin.size(), hin.data(), hout.size(), hout.data(),
-                     [&](exo_sht_plan_s* p, R* din, R* dout) {
-                       const R* ins[3] = {din, din + nhor, din + 2 * nhor};
-                       R* outs[3] = {dout, dout + nsp, dout + 2 * nsp};
-                       return run(p, ins, outs);
-                     });
-  if (rc) return rc;
-  for (int o = 0; o < nout; ++o) memcpy(out[o], hout.data() + o * nsp, sizeof(R) * nsp);
-  return EXO_OK;
+  return with_plan([&](exo_sht_plan_s* pl) {
+    const Tables& t = pl->tb;
+    const long nhor = (long)t.nlat * t.nlon * pl->nlev;
+    const long nsp = 2L * t.ncsp * pl->nlev;
+    std::vector<R> hin(nhor * nin), hout(nsp * nout);
+    for (int i = 0; i < nin; ++i) memcpy(hin.data() + i * nhor, in[i], sizeof(R) * nhor);
+    int rc = staged<R>(pl, hin.size(), hin.data(), hout.size(), hout.data(),
+                       [&](exo_sht_plan_s* p, R* din, R* dout) {
+                         const R* ins[3] = {din, din + nhor, din + 2 * nhor};
+                         R* outs[3] = {dout, dout + nsp, dout + 2 * nsp};
+                         return run(p, ins, outs);
+                       });
+    if (rc) return rc;
+    for (int o = 0; o < nout; ++o) memcpy(out[o], hout.data() + o * nsp, sizeof(R) * nsp);
+    return EXO_OK;
+  });
 }
 
 // uv2dv(pu, pv, pd, pz), src/legmod.f90:356 (folded branch)

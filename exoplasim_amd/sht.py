@@ -145,6 +145,9 @@ def dv2uv(pd: np.ndarray, pz: np.ndarray):
     nlev = _state["nlev"]
     pd = np.ascontiguousarray(pd)
     pz = np.ascontiguousarray(pz, dtype=pd.dtype)
+    for name, a in (("pd", pd), ("pz", pz)):
+        if a.size != nlev * 2 * ncsp:
+            raise ExoError(f"dv2uv: {name} holds {a.size} values, NLEV*NRSP = {nlev * 2 * ncsp}")
     pu = np.zeros((nlev, nlat, nlon), dtype=pd.dtype)
     pv = np.zeros((nlev, nlat, nlon), dtype=pd.dtype)
     _call("dv2uv", pd, pz, pu, pv)
@@ -223,6 +226,23 @@ def _stream():
     return torch.cuda.current_stream().cuda_stream
 
 
+def _fields(what, t, dtype, tail, nmem, *runs):
+    """Check one buffer of a batched call before its address reaches a
+    kernel: element type, rank, trailing shape, at least nmem members, and
+    every run (first, count) of field-levels inside shape[1]."""
+    if t.dtype != dtype:
+        raise ExoError(f"{what}: {t.dtype} buffer, the mode computes on {dtype}")
+    if t.dim() != 2 + len(tail) or tuple(t.shape[2:]) != tuple(tail):
+        raise ExoError(f"{what}: shape {tuple(t.shape)}, expected (nmem, nfield, "
+                       f"{', '.join(str(x) for x in tail)})")
+    if nmem < 0 or nmem > t.shape[0]:
+        raise ExoError(f"{what}: {nmem} members requested, buffer holds {t.shape[0]}")
+    for first, count in runs:
+        if first < 0 or count < 0 or first + count > t.shape[1]:
+            raise ExoError(f"{what}: field-levels [{first}, {first + count}) outside "
+                           f"the buffer's {t.shape[1]}")
+
+
 class Plan:
     """Transforms of one truncation with FP64 tables built on the GPU."""
 
@@ -271,12 +291,20 @@ class Plan:
         [src0, src0+nfield) of every member into grid fields [dst0, ...).
         sp: (nmem, nfl, nrsp), gp: (nmem, ngp, nlat, nlon)."""
         nmem = sp.shape[0] if nmem is None else nmem
+        if table not in ("qi", "qj"):
+            raise ExoError(f"inverse: table {table!r}, expected 'qi' or 'qj'")
+        dt = self.dtype_of(mode)
+        _fields("inverse sp", sp, dt, (self.nrsp,), nmem, (src0, nfield))
+        _fields("inverse gp", gp, dt, (self.nlat, self.nlon), nmem, (dst0, nfield))
         check(lib().exo_sht_inverse(self._h, mode_id(mode), TABLES.index(table), nmem,
                                     _ptr(sp), sp[0].numel(), src0, _ptr(gp), gp[0].numel(),
                                     dst0, nfield, _stream()), "inverse")
 
     def inverse_uv(self, mode, sp, gp, nlev, vor0, div0, u0, v0, nmem=None):
         nmem = sp.shape[0] if nmem is None else nmem
+        dt = self.dtype_of(mode)
+        _fields("inverse_uv sp", sp, dt, (self.nrsp,), nmem, (vor0, nlev), (div0, nlev))
+        _fields("inverse_uv gp", gp, dt, (self.nlat, self.nlon), nmem, (u0, nlev), (v0, nlev))
         check(lib().exo_sht_inverse_uv(self._h, mode_id(mode), nmem, _ptr(sp), sp[0].numel(),
                                        vor0, div0, _ptr(gp), gp[0].numel(), u0, v0, nlev,
                                        _stream()), "inverse_uv")
@@ -285,19 +313,42 @@ class Plan:
         """gp2fc + fc2sp of grid fields [src0, src0+nfield) into spectral
         field-levels [dst0, ...)."""
         nmem = gp.shape[0] if nmem is None else nmem
+        dt = self.dtype_of(mode)
+        _fields("direct gp", gp, dt, (self.nlat, self.nlon), nmem, (src0, nfield))
+        _fields("direct sp", sp, dt, (self.nrsp,), nmem, (dst0, nfield))
         check(lib().exo_sht_direct(self._h, mode_id(mode), nmem, _ptr(gp), gp[0].numel(),
                                    src0, _ptr(sp), sp[0].numel(), dst0, nfield, _stream()),
               "direct")
 
+    def _roundtrip_buffers(self, what, mode, nlev, sp_in, gp, sp_out=None):
+        """(nmem, 3*nlev+1, nrsp) spectral and (nmem, 5*nlev+2, nlat, nlon)
+        grid buffers of one round trip, all with the same nmem."""
+        if nlev < 1:
+            raise ExoError(f"{what}: nlev = {nlev}")
+        dt = self.dtype_of(mode)
+        nfl = 3 * nlev + 1
+        nmem = sp_in.shape[0] if sp_in.dim() else -1
+        bufs = [("sp_in", sp_in, (self.nrsp,), nfl), ("gp", gp, (self.nlat, self.nlon), nfl + 1 + 2 * nlev)]
+        if sp_out is not None:
+            bufs.append(("sp_out", sp_out, (self.nrsp,), nfl))
+        for name, t, tail, nf in bufs:
+            _fields(f"{what} {name}", t, dt, tail, nmem, (0, nf))
+            if t.shape[0] != nmem or t.shape[1] != nf:
+                raise ExoError(f"{what} {name}: shape {tuple(t.shape)}, expected "
+                               f"({nmem}, {nf}, {', '.join(str(x) for x in tail)})")
+        return nmem
+
     def roundtrip(self, mode, sp_in, gp, sp_out, nlev):
         """One round trip of every member (see exo_sht_roundtrip)."""
-        check(lib().exo_sht_roundtrip(self._h, mode_id(mode), sp_in.shape[0], nlev,
+        nmem = self._roundtrip_buffers("roundtrip", mode, nlev, sp_in, gp, sp_out)
+        check(lib().exo_sht_roundtrip(self._h, mode_id(mode), nmem, nlev,
                                       _ptr(sp_in), _ptr(gp), _ptr(sp_out), _stream()),
               "roundtrip")
 
     def inverse_all(self, mode, sp_in, gp, nlev):
         """The inverse half of a round trip in one launch (exo_sht_inverse_all)."""
-        check(lib().exo_sht_inverse_all(self._h, mode_id(mode), sp_in.shape[0], nlev,
+        nmem = self._roundtrip_buffers("inverse_all", mode, nlev, sp_in, gp)
+        check(lib().exo_sht_inverse_all(self._h, mode_id(mode), nmem, nlev,
                                         _ptr(sp_in), _ptr(gp), _stream()), "inverse_all")
 
     def set_tiles(self, mode, inv_var=0, uv_var=0, fwd_var=0):

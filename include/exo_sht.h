@@ -183,7 +183,7 @@ int exo_sht_autotune(exo_sht_plan_t plan, int mode, int nmem, int nlev, void *st
 /* No counterpart in the reference, which runs one model per process: an
  * ensemble of members with different truncations, each group in the layout
  * of exo_sht_roundtrip (sp_in and sp_out (nmem, 3*nlev+1, 2*NCSP), gp
- * (nmem, 4*nlev+2, NLAT, NLON), float device arrays).  Truncations 21, 42
+ * (nmem, 5*nlev+2, NLAT, NLON), float device arrays).  Truncations 21, 42
  * and 63; modes EXO_F32, EXO_F32MFMA and EXO_BF16X3.  The round trips of all
  * members run as two persistent kernels (all inverse work items, then all
  * direct work items) captured in one hipGraph at create time; the arrays

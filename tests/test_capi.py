@@ -3,6 +3,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -47,3 +48,72 @@ def test_no_fallback_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
     with pytest.raises(_lib.ExoError):
         _lib.lib()
+
+
+def _fake_plan(ntru=42):
+    """A Plan object with the dimensions of ntru and no device state, for
+    the checks that run before any address reaches the library."""
+    from exoplasim_amd import sht
+
+    p = sht.Plan.__new__(sht.Plan)
+    p._h = None
+    p.ntru = ntru
+    p.nlat, p.nlon, p.ncsp = sht.dims(ntru)
+    p.nrsp = 2 * p.ncsp
+    return p
+
+
+def test_batched_buffer_checks():
+    # wrong element type, shape, member count or field range raise ExoError
+    # before a device address is taken; right buffers on the host reach
+    # the device check
+    import torch
+    from exoplasim_amd import sht
+
+    p = _fake_plan(42)
+    nlev, nmem = 20, 3
+    nfl, ngp = 3 * nlev + 1, 5 * nlev + 2
+    sp32 = torch.zeros((nmem, nfl, p.nrsp), dtype=torch.float32)
+    gp32 = torch.zeros((nmem, ngp, p.nlat, p.nlon), dtype=torch.float32)
+    sp64 = sp32.double()
+
+    def raises(fn, text):
+        with pytest.raises(sht.ExoError, match=text):
+            fn()
+
+    # float32 buffers of a split-BF16 round trip handed to the FP64 mode
+    raises(lambda: p.roundtrip("f64", sp32, gp32, sp32.clone(), nlev), "float64")
+    raises(lambda: p.roundtrip("bf16x3", sp64, gp32, sp32, nlev), "float32")
+    # buffers of another level count or truncation
+    raises(lambda: p.roundtrip("bf16x3", sp32, gp32, sp32.clone(), nlev + 1), "outside")
+    raises(lambda: p.roundtrip("bf16x3", sp32[:, :, :100].contiguous(), gp32, sp32, nlev),
+           "shape")
+    raises(lambda: p.roundtrip("bf16x3", sp32, gp32[:2], sp32, nlev), "members")
+    raises(lambda: p.inverse_all("bf16x3", sp32, gp32, 0), "nlev")
+    # field ranges past the buffer and member counts above it
+    raises(lambda: p.inverse("bf16x3", sp32, gp32, 10, src0=nfl - 5), "outside")
+    raises(lambda: p.inverse("bf16x3", sp32, gp32, 10, dst0=ngp - 1), "outside")
+    raises(lambda: p.inverse("bf16x3", sp32, gp32, 1, nmem=4), "members")
+    raises(lambda: p.inverse("bf16x3", sp32, gp32, 1, table="qc"), "table")
+    raises(lambda: p.inverse_uv("f32", sp32, gp32, nlev, 0, nfl - 3, 0, nlev), "outside")
+    raises(lambda: p.inverse_uv("f32", sp32, gp32, nlev, 0, nlev, ngp - 5, 0), "outside")
+    raises(lambda: p.direct("f32", gp32, sp32, nfl, src0=0, dst0=1), "outside")
+    raises(lambda: p.direct("f32mfma", gp32, sp32, -1), "outside")
+    # consistent buffers get as far as the device check
+    raises(lambda: p.roundtrip("bf16x3", sp32, gp32, sp32.clone(), nlev), "device")
+    raises(lambda: p.roundtrip("f64", sp64, gp32.double(), sp64.clone(), nlev), "device")
+    raises(lambda: p.inverse("f32", sp32, gp32, nfl), "device")
+    raises(lambda: p.direct("f32", gp32, sp32, nfl), "device")
+
+
+def test_dropin_size_checks():
+    from exoplasim_amd import sht
+
+    saved = dict(sht._state)
+    try:
+        sht._state.update(ntru=21, nlat=32, nlev=10)
+        short = np.zeros((9, 2 * 253))
+        with pytest.raises(sht.ExoError, match="NLEV"):
+            sht.dv2uv(short, np.zeros((10, 2 * 253)))
+    finally:
+        sht._state.update(saved)

@@ -119,6 +119,49 @@ def test_ensemble_tiles_and_autotune(oracle, gpu):
         ens.close()
 
 
+FULL = [(21, 10, 48), (42, 20, 32), (63, 20, 16)]
+
+
+def test_ensemble_full_size_every_member(oracle, gpu):
+    # the 96-member ensemble at its benchmark size: every member of every
+    # group against the FP64 oracle, field by field, after an autotune
+    import torch
+
+    print(f"ensemble {FULL} bf16x3, every member", flush=True)
+    ens = sht.Ensemble(FULL, "bf16x3")
+    try:
+        host = []
+        for k, (ntru, nlev, nmem) in enumerate(FULL):
+            sp = ensemble(ntru, nlev, nmem, cfg=3)
+            ens.bufs[k][0].copy_(torch.from_numpy(sp).float())
+            host.append(sp)
+        ens.autotune()
+        _poison(ens)
+        ens.run()
+        out = _outputs(ens)
+        for k, (ntru, nlev, nmem) in enumerate(FULL):
+            tab = oracle.legini(ntru, sht.dims(ntru)[0])["_all"]
+            nfl = 3 * nlev + 1
+            fields = {"vor": slice(0, nlev), "div": slice(nlev, 2 * nlev),
+                      "tmp": slice(2 * nlev, 3 * nlev), "lnps": slice(nfl - 1, nfl)}
+            grid_only = {"dmu": slice(nfl, nfl + 1), "u": slice(nfl + 1, nfl + 1 + nlev),
+                         "v": slice(nfl + 1 + nlev, nfl + 1 + 2 * nlev)}
+            worst = 0.0
+            for m in range(nmem):
+                g_o, s_o = oracle.roundtrip(ntru, nlev, host[k][m], tables=tab)
+                for name, sl in {**fields, **grid_only}.items():
+                    e = rel_l2(out[k][0][m][sl], g_o[sl])
+                    assert e < 1e-6, (ntru, m, name, "grid", e)
+                    worst = max(worst, e)
+                for name, sl in fields.items():
+                    e = rel_l2(out[k][1][m][sl], s_o[sl])
+                    assert e < 1e-6, (ntru, m, name, "spectral", e)
+                    worst = max(worst, e)
+            print(f"  T{ntru}: {nmem} members, worst per-field rel L2 {worst:.2e}", flush=True)
+    finally:
+        ens.close()
+
+
 def test_ensemble_rejects(gpu):
     with pytest.raises(sht.ExoError):
         sht.Ensemble([(127, 4, 1)], "bf16x3")
