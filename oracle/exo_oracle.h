@@ -94,6 +94,18 @@ void ora_si_apply(int nlev, int nsp, const int *nindex, double delt, const doubl
                   const double *atm, const double *adm, double *stt, double *sdt,
                   double *spt);
 
+/* ---- grid-point dynamics (exo_oracle_dyn.c) ---- */
+void ora_initsi(int nlev, const double *sigmah, const double *dsigma, const double *rdsig,
+                const double *t0, double akap, double *g, double *tau, double *c,
+                double *t01s2, double *tkp);
+void ora_calcgp(int nlev, long nhor, const double *gu, const double *gv, const double *gt,
+                const double *gd, const double *gz, const double *gq, const double *gp,
+                const double *gpj, const double *gpm, const double *dq, const double *rcsq,
+                const double *sigmah, const double *dsigma, const double *rdsig,
+                const double *t0, const double *t01s2, const double *tkp, const double *c,
+                const double *g, const double *scal, int nqspec, double *gtn, double *gqn,
+                double *guz, double *gvz, double *gvp, double *gphi, double *dw);
+
 /* ---- long-wave radiation column (src/radmod.f90:2122-2456) ----
  * arrays [col][lev]; dt has nlev+1 values per column (surface last),
  * dq/dqo3/dqco2/dcc/dql nlev; dftu/dftd out nlev+1 */

@@ -26,6 +26,8 @@ _dp = POINTER(c_double)
 TABLES = ("qi", "qj", "qc", "qe", "qm", "qq", "qu", "qv")
 # plavor as the model holds it (src/plasimmod.f90:118,227: plavor = EZ)
 PLAVOR = 1.63299310207
+# kappa = R / cp of the Earth set-up (src/p_earth.f90:41-58)
+AKAP_EARTH = 0.286
 
 
 def _p(a: np.ndarray):
@@ -219,6 +221,49 @@ class Oracle:
                             _p(gcm), _p(tcm), _p(bm1))
         return np.transpose(bm1, (0, 2, 1)).copy()  # bm1[jn][row][col]
 
+    # -- grid-point dynamics (exo_oracle_dyn.c) ---------------------------
+    def initsi(self, sigmah, dsigma, rdsig, t0, akap=AKAP_EARTH):
+        """initsi's matrices; g, tau, c returned in math (row, col) order."""
+        n = len(t0)
+        c = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+        g, tau, cm = (np.zeros((n, n)) for _ in range(3))
+        t01s2, tkp = np.zeros(n), np.zeros(n)
+        self.lib.ora_initsi(c_int(n), _p(c(sigmah)), _p(c(dsigma)), _p(c(rdsig)), _p(c(t0)),
+                            c_double(akap), _p(g), _p(tau), _p(cm), _p(t01s2), _p(tkp))
+        return {"g": g.T.copy(), "tau": tau.T.copy(), "c": cm.T.copy(), "t01s2": t01s2,
+                "tkp": tkp}
+
+    def vertical(self, nlev, t0=None, akap=AKAP_EARTH):
+        """The vertical set-up of the dynamics: neqsig = 0 sigma levels,
+        rdsig = 0.5 / dsigma (src/plasim.f90:1553-1567), a t0 profile
+        (default 250 K, src/plasimmod.f90:593) and initsi's matrices."""
+        s, sh, ds = sigma_levels(self.lib, nlev)
+        t0 = np.full(nlev, 250.0) if t0 is None else np.asarray(t0, dtype=np.float64)
+        v = {"sigma": s, "sigmah": sh, "dsigma": ds, "rdsig": 0.5 / ds, "t0": t0,
+             "akap": akap}
+        v.update(self.initsi(sh, ds, v["rdsig"], t0, akap))
+        return v
+
+    def calcgp(self, f, v, scal, nqspec=1):
+        """calcgp on grid fields f (gu gv gt gd gz gq dq: (nlev, nhor); gp,
+        gpj, gpm, rcsq: (nhor,)), vertical set-up v (vertical()) and scal =
+        (rdbrv, adv, psurf, ww).  Returns gtn gqn guz gvz gphi dw (nlev, nhor)
+        and gvp (nhor,)."""
+        nlev, nhor = f["gu"].shape
+        c = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+        out = {k: np.zeros((nlev, nhor)) for k in ("gtn", "gqn", "guz", "gvz", "gphi", "dw")}
+        out["gvp"] = np.zeros(nhor)
+        sc = c([v["akap"], *scal])
+        self.lib.ora_calcgp(c_int(nlev), c_long(nhor), *[_p(c(f[k])) for k in
+                            ("gu", "gv", "gt", "gd", "gz", "gq", "gp", "gpj", "gpm", "dq",
+                             "rcsq")],
+                            *[_p(c(v[k])) for k in ("sigmah", "dsigma", "rdsig", "t0",
+                                                     "t01s2", "tkp")],
+                            _p(c(np.asarray(v["c"]).T)), _p(c(np.asarray(v["g"]).T)), _p(sc),
+                            c_int(nqspec), *[_p(out[k]) for k in
+                                             ("gtn", "gqn", "guz", "gvz", "gvp", "gphi", "dw")])
+        return out
+
 
 # ---------------------------------------------------------------------------
 # the reference's own Fortran, compiled into oracle/_ref (optional)
@@ -343,6 +388,79 @@ class RefPyfft:
                         byref(c_int(2 * nlat)), byref(c_int(ntru)), byref(c_int(1)),
                         byref(c_int(0)))
         return sp
+
+
+class RefPlasim:
+    """The reference's plasim.f90 at its default resolution module (T21 L10,
+    src/resmod_def.f90), FP64: initsi, makebm, minvers, the semi-implicit
+    part of spectrala and calcgp, through oracle/ref_glue_plasim.f90.
+    Loaded with lazy binding: plasim.f90 references model subroutines this
+    library does not link, none of which these calls reach."""
+
+    def __init__(self):
+        self.lib = ctypes.CDLL(_ref_path("libref_plasim.so"), mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+        d = (c_int * 8)()
+        self.lib.ref_plasim_dims(d)
+        (self.nlat, self.nlon, self.ntru, self.nlev, self.ncsp, self.nesp, self.nhor,
+         self.nspp) = list(d)
+
+    @staticmethod
+    def _c(a):
+        return np.ascontiguousarray(a, dtype=np.float64)
+
+    def initsi(self, sigma, sigmah, dsigma, rdsig, t0, akap=AKAP_EARTH):
+        n = self.nlev
+        g, tau, cm = (np.zeros((n, n)) for _ in range(3))
+        t01s2, tkp = np.zeros(n), np.zeros(n)
+        c = self._c
+        self.lib.ref_plasim_initsi(_p(c(sigma)), _p(c(sigmah)), _p(c(dsigma)), _p(c(rdsig)),
+                                   _p(c(t0)), byref(c_double(akap)), _p(g), _p(tau), _p(cm),
+                                   _p(t01s2), _p(tkp))
+        return {"g": g.T.copy(), "tau": tau.T.copy(), "c": cm.T.copy(), "t01s2": t01s2,
+                "tkp": tkp}
+
+    def makebm(self, delt, t0, dsigma, g, tau):
+        """bm1[jn][row][col] as Oracle.makebm; g, tau in math order."""
+        n, nt = self.nlev, self.ntru
+        bm1 = np.zeros((nt, n, n))
+        c = self._c
+        self.lib.ref_plasim_makebm(byref(c_double(delt)), _p(c(t0)), _p(c(dsigma)),
+                                   _p(c(np.asarray(g).T)), _p(c(np.asarray(tau).T)), _p(bm1))
+        return np.transpose(bm1, (0, 2, 1)).copy()
+
+    def minvers(self, a):
+        flat = np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)
+        self.lib.minvers_(_p(flat), byref(c_int(a.shape[0])))
+        return flat.T.copy()
+
+    def si_apply(self, delt, t0, dsigma, g, tau, bm1, sop, spm, atm, adm, stt, sdt, spt):
+        """spectrala sections 1-3 (arguments as Oracle.si_apply, spectral
+        arrays (nlev, nsp)); returns (stt, sdt, spt)."""
+        c = self._c
+        stt, sdt, spt = c(stt).copy(), c(sdt).copy(), c(spt).copy()
+        self.lib.ref_plasim_si(byref(c_double(delt)), _p(c(t0)), _p(c(dsigma)),
+                               _p(c(np.asarray(g).T)), _p(c(np.asarray(tau).T)),
+                               _p(c(np.transpose(bm1, (0, 2, 1)))), _p(c(sop)), _p(c(spm)),
+                               _p(c(atm)), _p(c(adm)), _p(stt), _p(sdt), _p(spt))
+        return stt, sdt, spt
+
+    def calcgp(self, f, v, scal, nqspec=1):
+        """calcgp with the arguments of Oracle.calcgp at (NLEV, NHOR); dq
+        has NLEV levels here and is padded to the model's NLEP."""
+        n, nh = self.nlev, self.nhor
+        c = self._c
+        out = {k: np.zeros((n, nh)) for k in ("gtn", "gqn", "guz", "gvz", "gphi", "dw")}
+        out["gvp"] = np.zeros(nh)
+        dq = np.zeros((n + 1, nh))
+        dq[:n] = f["dq"]
+        self.lib.ref_plasim_calcgp(*[_p(c(f[k])) for k in ("gu", "gv", "gt", "gd", "gz", "gq",
+                                                             "gp", "gpj", "gpm")],
+                                   _p(dq), _p(c(f["rcsq"])),
+                                   *[_p(c(v[k])) for k in ("sigmah", "dsigma", "rdsig", "t0")],
+                                   byref(c_double(v["akap"])), _p(c(scal)), c_int(nqspec),
+                                   *[_p(out[k]) for k in ("gtn", "gqn", "guz", "gvz", "gvp",
+                                                          "gphi", "dw")])
+        return out
 
 
 class RefFft991:

@@ -74,3 +74,23 @@ subroutine ref_t21_roundtrip(pspin, pgp, pspout) bind(C, name="ref_t21_roundtrip
     call fc2sp(zw, pspout(1, jf))
   enddo
 end subroutine
+
+! physics filter of legini (src/legmod.f90:123-155): set the namelist
+! switches, then run legini again; the legmod filters out as
+! skgpsp(NTRU+1), skspgp(NTRU+1)
+subroutine ref_t21_set_filter(kfilter, kgpt, kspv, pkappa, kexp, pn0, pgpsp, pspgp) &
+    bind(C, name="ref_t21_set_filter")
+  use legmod
+  integer(4), value :: kfilter, kgpt, kspv, kexp
+  real(8), value :: pkappa, pn0
+  real(8) :: pgpsp(NTP1), pspgp(NTP1)
+  nfilter = kfilter
+  ngptfilter = kgpt
+  nspvfilter = kspv
+  filterkappa = pkappa
+  nfilterexp = kexp
+  landhoskn0 = pn0
+  call legini
+  pgpsp(:) = skgpsp(:)
+  pspgp(:) = skspgp(:)
+end subroutine

@@ -80,7 +80,106 @@ def main():
     path = os.path.join(HERE, "reference_t21_t42.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes")
+    filter_golden(r, sp, nl)
+    plasim_golden()
     lwr_golden()
+
+
+# physics filters of legini (src/legmod.f90:123-155): nfilter 1..4 with both
+# directions switched on and the namelist defaults of the other parameters
+FILTER_PARAMS = dict(kappa=8.0, nexp=8, n0=15.0)
+
+
+def filter_golden(r, sp, nl):
+    from ctypes import c_double, c_int
+
+    from oracle import _p
+
+    out = {}
+    for nf in (1, 2, 3, 4):
+        g = np.zeros(r.ntru + 1)
+        s = np.zeros(r.ntru + 1)
+        r.lib.ref_t21_set_filter(c_int(nf), c_int(1), c_int(1), c_double(FILTER_PARAMS["kappa"]),
+                                 c_int(FILTER_PARAMS["nexp"]), c_double(FILTER_PARAMS["n0"]),
+                                 _p(g), _p(s))
+        out[f"f{nf}_skgpsp"], out[f"f{nf}_skspgp"] = g, s
+        fc = r.sp2fc(sp[0])
+        out[f"f{nf}_sp2fc"] = fc
+        out[f"f{nf}_sp2fcdmu"] = r.sp2fcdmu(sp[-1])
+        out[f"f{nf}_fc2sp"] = r.fc2sp(fc)
+        pu, pv = r.dv2uv(sp[nl:2 * nl], sp[:nl])
+        out[f"f{nf}_dv2uv_u0"], out[f"f{nf}_dv2uv_v0"] = pu[0], pv[0]
+        f = [golden_input((nl, r.nlat, r.nlon), 2.1 + 0.5 * k) for k in range(6)]
+        d, t, z = r.mktend(*f)
+        out[f"f{nf}_mktend_d0"], out[f"f{nf}_mktend_z0"] = d[0], z[0]
+    # back to the default (no filter) for later users of this library
+    g = np.zeros(r.ntru + 1)
+    r.lib.ref_t21_set_filter(c_int(0), c_int(0), c_int(0), c_double(8.0), c_int(8),
+                             c_double(15.0), _p(g), _p(g.copy()))
+    path = os.path.join(HERE, "reference_filters_t21.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes")
+
+
+# vertical set-up of the dynamics pins: a t0 profile from 220 K at the top
+# to 288 K at the bottom on the neqsig = 0 levels of the T21 L10 model
+def plasim_vertical(ora, nlev):
+    return ora.vertical(nlev, t0=np.linspace(220.0, 288.0, nlev))
+
+
+PLASIM_DELT = 0.05
+# (rdbrv, adv, psurf, ww) of the Earth set-up (src/p_earth.f90:41-58,
+# src/plasim.f90:1317-1318)
+PLASIM_SCAL = (287.0 / 461.5, 1870.0 / (287.0 / 0.286) - 1.0, 101100.0, 7.292e-5)
+
+
+def plasim_inputs(nlev, nhor, ntru):
+    """deterministic inputs of the semi-implicit and calcgp pins"""
+    nsp = (ntru + 1) * (ntru + 2)
+    si = {k: golden_input(shape, 0.37 * i + 0.1) for i, (k, shape) in enumerate(
+        (("sop", (nsp,)), ("spm", (nsp,)), ("atm", (nlev, nsp)), ("adm", (nlev, nsp)),
+         ("stt", (nlev, nsp)), ("sdt", (nlev, nsp)), ("spt", (nsp,))))}
+    gi = lambda shape, salt: golden_input(shape, salt)  # noqa: E731
+    f = {k: 0.1 * gi((nlev, nhor), 0.5 + 0.3 * i) for i, k in
+         enumerate(("gu", "gv", "gt", "gd", "gz"))}
+    f["gq"] = 1e-3 * np.abs(gi((nlev, nhor), 2.9))
+    f["dq"] = 1e-3 * gi((nlev, nhor), 3.3)
+    f["gp"] = np.exp(0.05 * gi((nhor,), 3.7))
+    f["gpj"] = 0.01 * gi((nhor,), 4.1)
+    f["gpm"] = 0.01 * gi((nhor,), 4.5)
+    f["rcsq"] = 1.0 / (1.0 - 0.98 * gi((nhor,), 4.9) ** 2)
+    return si, f
+
+
+CALCGP_KEEP = 16  # every 16th grid point of the calcgp outputs is stored
+
+
+def plasim_golden():
+    from oracle import Oracle, RefPlasim
+
+    r = RefPlasim()
+    ora = Oracle()
+    v = plasim_vertical(ora, r.nlev)
+    out = {}
+    rv = r.initsi(v["sigma"], v["sigmah"], v["dsigma"], v["rdsig"], v["t0"], v["akap"])
+    for k in ("g", "tau", "c", "t01s2", "tkp"):
+        out["initsi_" + k] = rv[k]
+    bm1 = r.makebm(PLASIM_DELT, v["t0"], v["dsigma"], rv["g"], rv["tau"])
+    out["bm1"] = bm1
+    a = golden_input((12, 12), 0.77) + 4.0 * np.eye(12)
+    out["minvers_12"] = r.minvers(a)
+    si, f = plasim_inputs(r.nlev, r.nhor, r.ntru)
+    stt, sdt, spt = r.si_apply(PLASIM_DELT, v["t0"], v["dsigma"], rv["g"], rv["tau"], bm1,
+                               si["sop"], si["spm"], si["atm"], si["adm"], si["stt"],
+                               si["sdt"], si["spt"])
+    out["si_stt"], out["si_sdt"], out["si_spt"] = stt, sdt, spt
+    for nq in (1, 0):
+        o = r.calcgp(f, {**v, **rv}, PLASIM_SCAL, nq)
+        for k, x in o.items():
+            out[f"calcgp{nq}_{k}"] = x[..., ::CALCGP_KEEP]
+    path = os.path.join(HERE, "reference_plasim_t21.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes")
 
 
 # parameter sets of the long-wave fixture: namelist defaults, then grey
@@ -125,5 +224,7 @@ def lwr_golden():
 if __name__ == "__main__":
     if sys.argv[1:] == ["lwr"]:
         lwr_golden()
+    elif sys.argv[1:] == ["plasim"]:
+        plasim_golden()
     else:
         main()
