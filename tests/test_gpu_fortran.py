@@ -76,3 +76,60 @@ def test_fortran_dropin(oracle, gpu, tmp_path, tag, ntru, nlat, nlev, tol):
     g = oracle.fc2gp(r["fc0"].reshape(nlat, nlon))
     assert rel_l2(r["g"], g) < tol
     assert rel_l2(r["g2"], oracle.gp2fc(r["g"].reshape(nlat, nlon))) < tol
+
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref")
+
+
+@pytest.mark.parametrize("tag,nf,nq,tol", [("r8", 2, 1, 1e-13), ("r8", 4, 0, 1e-13),
+                                           ("r8", 0, 1, 1e-13), ("r4", 3, 1, 2e-6)])
+def test_fortran_model_composites(oracle, gpu, tmp_path, tag, nf, nq, tol):
+    """legini (with physics filter nf on both directions), invlega, invlegd,
+    sp3fc and sp2fl called by their reference names from a program built
+    against the reference's pumamod (T21 L10): every pumamod array they fill
+    against the oracle on the program's own spectral state."""
+    exe = os.path.join(REF_BIN, f"driver_model_{tag}")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built (make -C oracle ref, needs the reference tree)")
+    out = tmp_path / "model.bin"
+    print(f"fortran model driver {tag} nfilter={nf} nqspec={nq}", flush=True)
+    res = subprocess.run([exe, str(nf), str(nq), str(out)], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    ntru, nlat, nlon, nlev = 21, 32, 64, 10
+    nesp, nhor = (ntru + 1) * (ntru + 2), nlat * nlon
+    v = np.fromfile(out, dtype=np.float64)
+    layout = [("sd", nesp * nlev), ("st", nesp * nlev), ("sz", nesp * nlev),
+              ("sq", nesp * nlev), ("sp", nesp)]
+    layout += [(f"a_{k}", nhor * nlev) for k in ("gd", "gt", "gz", "gq", "gu", "gv")]
+    layout += [("a_gp", nhor), ("a_gpj", nhor)]
+    layout += [(f"d_{k}", nhor * nlev) for k in ("gt", "gq", "gu", "gv")] + [("d_gp", nhor)]
+    layout += [(f"s_{k}", nhor * nlev) for k in ("gd", "gt", "gz")] + [("fl_sq", nhor * nlev)]
+    assert v.size == sum(s for _, s in layout)
+    r, o = {}, 0
+    for name, s in layout:
+        shape = (nlev, nlat, nlon) if s == nhor * nlev else ((nlev, nesp) if s == nesp * nlev
+                                                             else (-1,))
+        r[name] = v[o:o + s].reshape(shape)
+        o += s
+    sid, gwd = oracle.inigau(nlat)
+    tab = oracle.legini(ntru, nlat, sid, gwd)
+    _, s = oracle.filters(ntru, nf, 1, 1)
+    fc = lambda sp, t="qi": oracle.sp2fc(ntru, nlat, nlon, tab[t], s, sp)  # noqa: E731
+    lev = lambda a: np.stack([fc(a[j]) for j in range(nlev)])  # noqa: E731
+    plavor = 1.63299310207 if tag == "r8" else float(np.float32(1.63299310207))
+    pu, pv = oracle.dv2uv(ntru, nlat, nlon, nlev, tab["qu"], tab["qv"], s, plavor, r["sd"],
+                          r["sz"])
+    want = {"a_gd": lev(r["sd"]), "a_gt": lev(r["st"]), "a_gz": lev(r["sz"]),
+            "a_gq": lev(r["sq"]) if nq == 1 else np.zeros((nlev, nlat, nlon)),
+            "a_gu": pu, "a_gv": pv, "a_gp": fc(r["sp"]).ravel(),
+            "a_gpj": fc(r["sp"], "qj").ravel(),
+            "d_gt": lev(r["st"]), "d_gq": lev(r["sq"]) if nq == 1 else np.zeros((nlev, nlat, nlon)),
+            "d_gu": pu, "d_gv": pv, "d_gp": fc(r["sp"]).ravel(),
+            "s_gd": lev(r["sd"]), "s_gt": lev(r["st"]), "s_gz": lev(r["sz"]),
+            "fl_sq": lev(r["sq"])}
+    for k, w in want.items():
+        if not np.any(w):
+            assert np.all(r[k] == 0.0), k
+        else:
+            assert rel_l2(r[k], w) < tol, (k, rel_l2(r[k], w))

@@ -71,17 +71,18 @@ def test_tables(oracle, gpu, ntru):
         assert err < 1e-14, (name, err)
 
 
-def _oracle_roundtrip(oracle, ntru, nlev, sp):
+def _oracle_roundtrip(oracle, ntru, nlev, sp, sk=None):
     gps, sps = [], []
+    tab = oracle.legini(ntru, sht.dims(ntru)[0])["_all"]
     for k in range(sp.shape[0]):
-        g, s = oracle.roundtrip(ntru, nlev, sp[k])
+        g, s = oracle.roundtrip(ntru, nlev, sp[k], tables=tab, sk=sk)
         gps.append(g)
         sps.append(s)
     return np.stack(gps), np.stack(sps)
 
 
 def _check_roundtrip(p, oracle, mode, ntru, nlev, nmem=2, cfg=2, slope=1.5, wave1=0.0,
-                     refs=None):
+                     refs=None, identity=True):
     import torch
 
     sp = ensemble(ntru, nlev, nmem, cfg=cfg, slope=slope, wave1=wave1)
@@ -113,8 +114,9 @@ def _check_roundtrip(p, oracle, mode, ntru, nlev, nmem=2, cfg=2, slope=1.5, wave
             assert e < tol, (k, name, "spectral", e)
     print("  per-field rel L2: " + ", ".join(f"{k} {v:.1e}" for k, v in worst.items()),
           flush=True)
-    # the round trip reproduces its band-limited input
-    assert rel_l2(s, sp) < max(tol, 1e-12) * 10
+    # the round trip reproduces its band-limited input (without filters)
+    if identity:
+        assert rel_l2(s, sp) < max(tol, 1e-12) * 10
 
 
 @pytest.mark.parametrize("mode", ["f64", "f32", "f32mfma", "bf16x3"])
@@ -143,6 +145,60 @@ def test_roundtrip_t127_l40_tidal(plans, oracle, gpu, mode):
                              refs=refs)
     finally:
         p.set_tiles(mode, 0, 0, 0)
+
+
+@pytest.mark.parametrize("nf", [1, 2, 3, 4])
+@pytest.mark.parametrize("mode", ["f64", "bf16x3", "f32mfma"])
+def test_roundtrip_physics_filters(oracle, gpu, nf, mode):
+    # legini's physics filters (src/legmod.f90:123-155: Cesaro, exponential,
+    # Lander-Hoskins, Riesz-2) on both directions, folded into the GPU
+    # tables: the T42 L20 round trip against the oracle with the same filters
+    ntru, nlev = 42, 20
+    print(f"roundtrip T{ntru} L{nlev} nfilter={nf} {mode}", flush=True)
+    g, s = oracle.filters(ntru, nf, 1, 1)
+    p = sht.Plan(ntru, skgpsp=g, skspgp=s)
+    try:
+        sp = ensemble(ntru, nlev, 2, cfg=2)
+        refs = _oracle_roundtrip(oracle, ntru, nlev, sp, sk=(g, s))
+        _check_roundtrip(p, oracle, mode, ntru, nlev, refs=refs, identity=False)
+    finally:
+        p.close()
+
+
+@pytest.mark.parametrize("nf", [1, 3])
+def test_dropin_physics_filters(oracle, gpu, nf):
+    # the drop-in entry points after legini with filters, FP64 (the model's
+    # MOST_PREC=8 build): sp2fc, sp2fcdmu, dv2uv take skspgp; fc2sp, uv2dv,
+    # mktend take skgpsp
+    print(f"drop-in T21 nfilter={nf}", flush=True)
+    ntru, nlev = 21, 10
+    nlat, nlon, _ = sht.dims(ntru)
+    g, s = oracle.filters(ntru, nf, 1, 1)
+    sht.legini(ntru, nlat, nlev, skgpsp=g, skspgp=s)
+    tab = oracle.legini(ntru, nlat)
+    sp = ensemble(ntru, nlev, 1, cfg=8)[0]
+    fc = sht.sp2fc(sp[:2])
+    for f in range(2):
+        assert rel_l2(fc[f], oracle.sp2fc(ntru, nlat, nlon, tab["qi"], s, sp[f])) < 1e-13
+    assert rel_l2(sht.sp2fcdmu(sp[-1])[0],
+                  oracle.sp2fc(ntru, nlat, nlon, tab["qj"], s, sp[-1])) < 1e-13
+    back = sht.fc2sp(fc)
+    for f in range(2):
+        assert rel_l2(back[f], oracle.fc2sp(ntru, nlat, nlon, tab["qc"], g, fc[f])) < 1e-13
+    pu, pv = sht.dv2uv(sp[nlev:2 * nlev], sp[:nlev])
+    ou, ov = oracle.dv2uv(ntru, nlat, nlon, nlev, tab["qu"], tab["qv"], s, sht.PLAVOR,
+                          sp[nlev:2 * nlev], sp[:nlev])
+    assert rel_l2(pu, ou) < 1e-13 and rel_l2(pv, ov) < 1e-13
+    rng = np.random.default_rng(nf)
+    f6 = [rng.standard_normal((nlev, nlat, nlon)) for _ in range(6)]
+    d, t, z = sht.mktend(*f6)
+    o = oracle.mktend(ntru, nlat, nlon, nlev, tab["qc"], tab["qe"], tab["qm"], tab["qq"], g, *f6)
+    for x, y in zip((d, t, z), o):
+        assert rel_l2(x, y) < 1e-13
+    pd, pz = sht.uv2dv(f6[0], f6[1])
+    od, oz = oracle.uv2dv(ntru, nlat, nlon, nlev, tab["qe"], tab["qm"], g, f6[0], f6[1])
+    assert rel_l2(pd, od) < 1e-13 and rel_l2(pz, oz) < 1e-13
+    sht.legini(ntru, nlat, nlev)  # back to no filter
 
 
 @pytest.mark.parametrize("mode", ["f64", "f32", "f32mfma", "bf16x3"])
