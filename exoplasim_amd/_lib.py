@@ -51,6 +51,19 @@ PROTOTYPES = {
     "exo_sht_roundtrip": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p,
                                   c_void_p, c_void_p]),
     "exo_sht_inverse_all": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "exo_sht_sp2fc": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_long, c_int,
+                              c_void_p, c_long, c_int, c_int, c_void_p]),
+    "exo_sht_dv2uv": (c_int, [c_void_p, c_int, c_int, c_void_p, c_long, c_int, c_int,
+                              c_void_p, c_long, c_int, c_int, c_int, c_void_p]),
+    "exo_sht_fc2sp": (c_int, [c_void_p, c_int, c_int, c_void_p, c_long, c_int, c_void_p,
+                              c_long, c_int, c_int, c_void_p]),
+    "exo_sht_fft": (c_int, [c_void_p, c_int, c_int, c_void_p, c_long, c_void_p]),
+    "exo_sht_uv2dv": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_long,
+                              c_void_p, c_void_p, c_long, c_void_p]),
+    "exo_sht_qtend": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_long, c_void_p,
+                              c_void_p, c_void_p, c_long, c_void_p]),
+    "exo_sht_mktend": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                               c_long] + [c_void_p] * 6 + [c_long, c_void_p]),
     "exo_sht_set_tiles": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     "exo_sht_get_tiles": (c_int, [c_void_p, c_int, POINTER(c_int)]),
     "exo_sht_autotune": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, POINTER(c_int)]),

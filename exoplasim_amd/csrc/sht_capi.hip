@@ -6,23 +6,9 @@
 #include <vector>
 
 #include "../../include/exo_sht.h"
-#include "sht_internal.h"
+#include "sht_plan.h"
 
 using namespace exo;
-
-struct exo_sht_plan_s {
-  Tables tb;
-  void* scratch = nullptr;  // partial direct sums of multi-block truncations
-  size_t scratch_bytes = 0;
-  void* stage = nullptr;    // device staging of the host drop-in calls
-  size_t stage_bytes = 0;
-  void* pk = nullptr;       // pre-split spectral operand of the split-BF16 inverse
-  size_t pk_bytes = 0;
-  int nlev = 0;
-  // tile variants per precision mode, chosen by exo_sht_autotune
-  int inv_var[4] = {0, 0, 0, 0}, uv_var[4] = {0, 0, 0, 0}, fwd_var[4] = {0, 0, 0, 0};
-  std::mutex mu;
-};
 
 #define HIPCK(x)                        \
   do {                                  \
@@ -153,7 +139,7 @@ static int copy_tables(const Tables& t, double* out) {
   return EXO_OK;
 }
 
-static int grow(void** p, size_t* have, size_t need) {
+int plan_grow(void** p, size_t* have, size_t need) {
   if (*have >= need) return EXO_OK;
   if (*p) (void)hipFree(*p);
   *p = nullptr;
@@ -218,17 +204,22 @@ static bool mode_ok(int mode) { return mode >= EXO_F64 && mode <= EXO_BF16X3; }
 
 // MODE_BF16X3: pre-split the fp32 spectral fields [f0, f0 + nf) of every
 // member into the plan's pack buffer and point a at it
+// (buf: a caller-given buffer of at least pack_bytes(nmem, nf) bytes, or
+// null for the plan's own, grown as needed)
 static int pack_for(exo_sht_plan_t p, int mode, int nmem, const void* sp, long sps, int f0,
-                    int nf, hipStream_t s, InvArgs& a) {
+                    int nf, hipStream_t s, InvArgs& a, void* buf = nullptr) {
   a.bpk = nullptr;
   a.bpk_stride = 0;
   a.bpk_f0 = 0;
   if (mode != MODE_BF16X3 || nmem == 0 || nf == 0) return EXO_OK;
-  int rc = grow(&p->pk, &p->pk_bytes, pack_bytes(p->tb, nmem, nf));
-  if (rc != EXO_OK) return rc;
-  if (launch_pack(p->tb, (const float*)sp, sps, f0, nf, nmem, (uint16_t*)p->pk, s) != 0)
+  if (!buf) {
+    int rc = plan_grow(&p->pk, &p->pk_bytes, pack_bytes(p->tb, nmem, nf));
+    if (rc != EXO_OK) return rc;
+    buf = p->pk;
+  }
+  if (launch_pack(p->tb, (const float*)sp, sps, f0, nf, nmem, (uint16_t*)buf, s) != 0)
     return EXO_ERR_LAUNCH;
-  a.bpk = p->pk;
+  a.bpk = buf;
   a.bpk_stride = (long)nf * pk_field(p->tb.ntru);
   a.bpk_f0 = f0;
   return EXO_OK;
@@ -237,7 +228,7 @@ static int pack_for(exo_sht_plan_t p, int mode, int nmem, const void* sp, long s
 // pk: an existing pack of the fields (round trip), or null to pack here
 static int inverse_impl(exo_sht_plan_t p, int mode, int table, int nmem, const void* sp,
                         long sp_stride, int src0, void* gp, long gp_stride, int dst0, int nfield,
-                        hipStream_t s, const InvArgs* pk) {
+                        hipStream_t s, const InvArgs* pk, bool fused = true) {
   InvArgs a{};
   a.sp = sp;
   a.out = gp;
@@ -259,12 +250,12 @@ static int inverse_impl(exo_sht_plan_t p, int mode, int table, int nmem, const v
     const int rc = pack_for(p, mode, nmem, sp, sp_stride, src0, nfield, s, a);
     if (rc != EXO_OK) return rc;
   }
-  return launch_inverse(p->tb, mode, true, a, s);
+  return launch_inverse(p->tb, mode, fused, a, s);
 }
 
 static int inverse_uv_impl(exo_sht_plan_t p, int mode, int nmem, const void* sp, long sp_stride,
                            int vor0, int div0, void* gp, long gp_stride, int u0, int v0, int nlev,
-                           hipStream_t s, const InvArgs* pk) {
+                           hipStream_t s, const InvArgs* pk, bool fused = true) {
   InvArgs a{};
   a.sp = sp;
   a.out = gp;
@@ -290,7 +281,7 @@ static int inverse_uv_impl(exo_sht_plan_t p, int mode, int nmem, const void* sp,
     const int rc = pack_for(p, mode, nmem, sp, sp_stride, lo, hi - lo, s, a);
     if (rc != EXO_OK) return rc;
   }
-  return launch_inverse(p->tb, mode, true, a, s);
+  return launch_inverse(p->tb, mode, fused, a, s);
 }
 
 extern "C" int exo_sht_inverse(exo_sht_plan_t p, int mode, int table, int nmem, const void* sp,
@@ -315,9 +306,11 @@ extern "C" int exo_sht_inverse_uv(exo_sht_plan_t p, int mode, int nmem, const vo
                          (hipStream_t)stream, nullptr);
 }
 
+// (scr, sbytes: a caller-given buffer for the partial sums of multi-block
+// truncations, or null for the plan's own, grown as needed)
 static int direct_impl(exo_sht_plan_t p, int mode, bool fused, int nmem, const void* in,
                        long in_stride, int src0, void* sp, long sp_stride, int dst0, int nfield,
-                       hipStream_t s) {
+                       hipStream_t s, void* scr = nullptr, size_t sbytes = 0) {
   FwdArgs a{};
   a.in = in;
   a.sp = sp;
@@ -329,9 +322,10 @@ static int direct_impl(exo_sht_plan_t p, int mode, bool fused, int nmem, const v
   a.nfield = nfield;
   a.table = TQC;
   a.var = fused ? p->fwd_var[mode] : 0;
+  if (scr) return launch_forward(p->tb, mode, fused, a, scr, sbytes, s);
   const size_t need = forward_scratch_bytes(p->tb, mode, nmem, nfield);
   if (need > 0) {
-    int rc = grow(&p->scratch, &p->scratch_bytes, need);
+    int rc = plan_grow(&p->scratch, &p->scratch_bytes, need);
     if (rc != EXO_OK) return rc;
   }
   return launch_forward(p->tb, mode, fused, a, p->scratch, p->scratch_bytes, s);
@@ -348,7 +342,7 @@ extern "C" int exo_sht_direct(exo_sht_plan_t p, int mode, int nmem, const void* 
 }
 
 static int inverse_all_impl(exo_sht_plan_t p, int mode, int nmem, int nlev, const void* sp_in,
-                            void* gp, hipStream_t s);
+                            void* gp, hipStream_t s, void* pkbuf = nullptr);
 
 extern "C" int exo_sht_roundtrip(exo_sht_plan_t p, int mode, int nmem, int nlev, const void* sp_in,
                                  void* gp, void* sp_out, void* stream) {
@@ -392,13 +386,13 @@ extern "C" int exo_sht_get_tiles(exo_sht_plan_t p, int mode, int* vars) {
 // the inverse half of exo_sht_roundtrip: pre-split (split-BF16) and the
 // three inverse transforms in one launch
 static int inverse_all_impl(exo_sht_plan_t p, int mode, int nmem, int nlev, const void* sp_in,
-                            void* gp, hipStream_t s) {
+                            void* gp, hipStream_t s, void* pkbuf) {
   const Tables& t = p->tb;
   const int nfl = 3 * nlev + 1;
   const long sps = (long)nfl * 2 * t.ncsp;
   const long gps = (long)(nfl + 1 + 2 * nlev) * t.nlat * t.nlon;
   InvArgs pk{};
-  int rc = pack_for(p, mode, nmem, sp_in, sps, 0, nfl, s, pk);
+  int rc = pack_for(p, mode, nmem, sp_in, sps, 0, nfl, s, pk, pkbuf);
   if (rc) return rc;
   InvArgs aq{};
   aq.sp = sp_in;
@@ -576,7 +570,7 @@ template <typename R, typename Fn>
 static int staged(exo_sht_plan_s* p, size_t in_elems, const R* in, size_t out_elems, R* out,
                   Fn run) {
   const size_t need = sizeof(R) * (in_elems + out_elems);
-  int rc = grow(&p->stage, &p->stage_bytes, need);
+  int rc = plan_grow(&p->stage, &p->stage_bytes, need);
   if (rc) return rc;
   R* din = (R*)p->stage;
   R* dout = din + in_elems;
@@ -719,11 +713,12 @@ static int dirx_one(exo_sht_plan_s* p, const R* const* din, int nin, R* dout,
   a.out = dout;
   a.nin = nin;
   a.nlev = p->nlev;
+  a.nmem = 1;
   a.nterm = 0;
   for (const DirxTerm& t : terms) a.terms[a.nterm++] = t;
   const size_t need = dirx_scratch_bytes(p->tb, mode, p->nlev);
   if (need > 0) {
-    int rc = grow(&p->scratch, &p->scratch_bytes, need);
+    int rc = plan_grow(&p->scratch, &p->scratch_bytes, need);
     if (rc) return rc;
   }
   return launch_dirx(p->tb, mode, a, p->scratch, p->scratch_bytes, 0);
@@ -841,4 +836,147 @@ extern "C" int exo_mktend_f32(float* d, float* t, float* z, const float* tn, con
                               const float* fv, const float* ke, const float* ut,
                               const float* vt) {
   return mktend_impl<float>(d, t, z, tn, fu, fv, ke, ut, vt);
+}
+
+// ---------------------------------------------------------------------------
+// device-resident Fourier-space transforms on plans (no fused FFT)
+// ---------------------------------------------------------------------------
+extern "C" int exo_sht_sp2fc(exo_sht_plan_t p, int mode, int table, int nmem, const void* sp,
+                             long sp_stride, int src0, void* fc, long fc_stride, int dst0,
+                             int nfield, void* stream) {
+  if (!p || !mode_ok(mode) || (table != EXO_QI && table != EXO_QJ) || nmem < 0 || nfield < 0 ||
+      src0 < 0 || dst0 < 0 || (nmem > 0 && nfield > 0 && (!sp || !fc)))
+    return EXO_ERR_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
+  return inverse_impl(p, mode, table, nmem, sp, sp_stride, src0, fc, fc_stride, dst0, nfield,
+                      (hipStream_t)stream, nullptr, false);
+}
+
+extern "C" int exo_sht_dv2uv(exo_sht_plan_t p, int mode, int nmem, const void* sp,
+                             long sp_stride, int vor0, int div0, void* fc, long fc_stride, int u0,
+                             int v0, int nlev, void* stream) {
+  if (!p || !mode_ok(mode) || nmem < 0 || nlev < 0 || vor0 < 0 || div0 < 0 || u0 < 0 ||
+      v0 < 0 || (nmem > 0 && nlev > 0 && (!sp || !fc)))
+    return EXO_ERR_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
+  return inverse_uv_impl(p, mode, nmem, sp, sp_stride, vor0, div0, fc, fc_stride, u0, v0, nlev,
+                         (hipStream_t)stream, nullptr, false);
+}
+
+extern "C" int exo_sht_fc2sp(exo_sht_plan_t p, int mode, int nmem, const void* fc,
+                             long fc_stride, int src0, void* sp, long sp_stride, int dst0,
+                             int nfield, void* stream) {
+  if (!p || !mode_ok(mode) || nmem < 0 || nfield < 0 || src0 < 0 || dst0 < 0 ||
+      (nmem > 0 && nfield > 0 && (!sp || !fc)))
+    return EXO_ERR_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
+  return direct_impl(p, mode, false, nmem, fc, fc_stride, src0, sp, sp_stride, dst0, nfield,
+                     (hipStream_t)stream);
+}
+
+extern "C" int exo_sht_fft(exo_sht_plan_t p, int mode, int dir, void* a, long lot,
+                           void* stream) {
+  if (!p || !mode_ok(mode) || (dir != 1 && dir != -1) || lot < 0 || (lot > 0 && !a))
+    return EXO_ERR_ARG;
+  if (lot == 0) return EXO_OK;
+  std::lock_guard<std::mutex> lk(p->mu);
+  const int rc = launch_fft(p->tb, mode, dir, a, p->tb.nlon, lot, (hipStream_t)stream);
+  return rc == 0 ? EXO_OK : (rc == -2 ? EXO_ERR_UNSUPPORTED : EXO_ERR_LAUNCH);
+}
+
+// one output field of a multi-term direct transform over nmem members x
+// nlev levels of device Fourier arrays
+static int dirx_dev(exo_sht_plan_t p, int mode, int nmem, int nlev, const void* const* in,
+                    int nin, long in_stride, void* out, long out_stride,
+                    std::initializer_list<DirxTerm> terms, hipStream_t s) {
+  DirxArgs a{};
+  for (int i = 0; i < nin; ++i) a.in[i] = in[i];
+  a.out = out;
+  a.nin = nin;
+  a.nlev = nlev;
+  a.nmem = nmem;
+  a.in_stride = in_stride;
+  a.out_stride = out_stride;
+  a.nterm = 0;
+  for (const DirxTerm& t : terms) a.terms[a.nterm++] = t;
+  const size_t need = dirx_scratch_bytes(p->tb, mode, nmem * nlev);
+  if (need > 0) {
+    const int rc = plan_grow(&p->scratch, &p->scratch_bytes, need);
+    if (rc) return rc;
+  }
+  const int rc = launch_dirx(p->tb, mode, a, p->scratch, p->scratch_bytes, s);
+  return rc == 0 ? EXO_OK : (rc == -4 ? EXO_ERR_SCRATCH : EXO_ERR_LAUNCH);
+}
+
+static bool all_set(std::initializer_list<const void*> ps) {
+  for (const void* q : ps)
+    if (!q) return false;
+  return true;
+}
+
+extern "C" int exo_sht_uv2dv(exo_sht_plan_t p, int mode, int nmem, int nlev, const void* pu,
+                             const void* pv, long f_stride, void* pd, void* pz, long s_stride,
+                             void* stream) {
+  if (!p || !mode_ok(mode) || nmem < 0 || nlev < 0 ||
+      (nmem > 0 && nlev > 0 && !all_set({pu, pv, pd, pz})))
+    return EXO_ERR_ARG;
+  if (nmem == 0 || nlev == 0) return EXO_OK;
+  std::lock_guard<std::mutex> lk(p->mu);
+  hipStream_t s = (hipStream_t)stream;
+  const void* in[2] = {pu, pv};
+  // pd = -qe.v + i qm.u ; pz = qe.u + i qm.v   (inputs 0 = u, 1 = v)
+  int rc = dirx_dev(p, mode, nmem, nlev, in, 2, f_stride, pd, s_stride,
+                    {{1, 0, 0, -1, TQE}, {0, 1, 0, -1, TQM}, {1, 1, 1, -1, TQE}, {0, 0, 1, +1, TQM}},
+                    s);
+  if (rc) return rc;
+  return dirx_dev(p, mode, nmem, nlev, in, 2, f_stride, pz, s_stride,
+                  {{0, 0, 0, +1, TQE}, {1, 1, 0, -1, TQM}, {0, 1, 1, +1, TQE}, {1, 0, 1, +1, TQM}},
+                  s);
+}
+
+extern "C" int exo_sht_qtend(exo_sht_plan_t p, int mode, int nmem, int nlev, void* q,
+                             long s_stride, const void* qn, const void* uq, const void* vq,
+                             long f_stride, void* stream) {
+  if (!p || !mode_ok(mode) || nmem < 0 || nlev < 0 ||
+      (nmem > 0 && nlev > 0 && !all_set({q, qn, uq, vq})))
+    return EXO_ERR_ARG;
+  if (nmem == 0 || nlev == 0) return EXO_OK;
+  std::lock_guard<std::mutex> lk(p->mu);
+  const void* in[3] = {qn, uq, vq};
+  // q = qe.vq + qc.qn - i qm.uq   (inputs 0 = qn, 1 = uq, 2 = vq)
+  return dirx_dev(p, mode, nmem, nlev, in, 3, f_stride, q, s_stride,
+                  {{2, 0, 0, +1, TQE}, {0, 0, 0, +1, TQC}, {1, 1, 0, +1, TQM},
+                   {2, 1, 1, +1, TQE}, {0, 1, 1, +1, TQC}, {1, 0, 1, -1, TQM}},
+                  (hipStream_t)stream);
+}
+
+extern "C" int exo_sht_mktend(exo_sht_plan_t p, int mode, int nmem, int nlev, void* d, void* t,
+                              void* z, long s_stride, const void* tn, const void* fu,
+                              const void* fv, const void* ke, const void* ut, const void* vt,
+                              long f_stride, void* stream) {
+  if (!p || !mode_ok(mode) || nmem < 0 || nlev < 0 ||
+      (nmem > 0 && nlev > 0 && !all_set({d, t, z, tn, fu, fv, ke, ut, vt})))
+    return EXO_ERR_ARG;
+  if (nmem == 0 || nlev == 0) return EXO_OK;
+  std::lock_guard<std::mutex> lk(p->mu);
+  hipStream_t s = (hipStream_t)stream;
+  // D = qq.ke - qe.fv + i qm.fu   (0 = ke, 1 = fv, 2 = fu)
+  const void* in_d[3] = {ke, fv, fu};
+  int rc = dirx_dev(p, mode, nmem, nlev, in_d, 3, f_stride, d, s_stride,
+                    {{0, 0, 0, +1, TQQ}, {1, 0, 0, -1, TQE}, {2, 1, 0, -1, TQM},
+                     {0, 1, 1, +1, TQQ}, {1, 1, 1, -1, TQE}, {2, 0, 1, +1, TQM}},
+                    s);
+  if (rc) return rc;
+  // T = qe.vt + qc.tn - i qm.ut   (0 = vt, 1 = tn, 2 = ut)
+  const void* in_t[3] = {vt, tn, ut};
+  rc = dirx_dev(p, mode, nmem, nlev, in_t, 3, f_stride, t, s_stride,
+                {{0, 0, 0, +1, TQE}, {1, 0, 0, +1, TQC}, {2, 1, 0, +1, TQM},
+                 {0, 1, 1, +1, TQE}, {1, 1, 1, +1, TQC}, {2, 0, 1, -1, TQM}},
+                s);
+  if (rc) return rc;
+  // Z = qe.fu + i qm.fv   (0 = fu, 1 = fv)
+  const void* in_z[2] = {fu, fv};
+  return dirx_dev(p, mode, nmem, nlev, in_z, 2, f_stride, z, s_stride,
+                  {{0, 0, 0, +1, TQE}, {1, 1, 0, -1, TQM}, {0, 1, 1, +1, TQE}, {1, 0, 1, +1, TQM}},
+                  s);
 }

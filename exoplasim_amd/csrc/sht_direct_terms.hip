@@ -38,8 +38,9 @@ __global__ __launch_bounds__(256) void k_dirx(DirxArgs a, const void* tabv, void
   __shared__ uint8_t mw[S::NCSP];
   const int tid = threadIdx.x;
   const int pb = blockIdx.x % NPB;
-  const int lev = blockIdx.x / NPB;
-  if (lev >= a.nlev) return;
+  const int ml = blockIdx.x / NPB;  // member * nlev + level
+  if (ml >= a.nmem * a.nlev) return;
+  const int mem = ml / a.nlev, lev = ml - mem * a.nlev;
   for (int m = tid; m <= T; m += 256)
     for (int n = m; n <= T; ++n) mw[w0_of(T, m) + n - m] = (uint8_t)m;
   auto xi = [](int i, int m, int cls, int pl, int ri) {
@@ -48,7 +49,7 @@ __global__ __launch_bounds__(256) void k_dirx(DirxArgs a, const void* tabv, void
   for (int idx = tid; idx < a.nin * PB * NM; idx += 256) {
     const int pl = idx % PB, k = (idx / PB) % NM, i = idx / (PB * NM);
     const int lat = pb * PB + pl;
-    const R* src = (const R*)a.in[i] + (long)lev * NLAT * N;
+    const R* src = (const R*)a.in[i] + mem * a.in_stride + (long)lev * NLAT * N;
     const R nx = src[(long)lat * N + 2 * k], ny = src[(long)lat * N + 2 * k + 1];
     const R sx = src[(long)(NLAT - 1 - lat) * N + 2 * k];
     const R sy = src[(long)(NLAT - 1 - lat) * N + 2 * k + 1];
@@ -62,9 +63,9 @@ __global__ __launch_bounds__(256) void k_dirx(DirxArgs a, const void* tabv, void
   const long tsz = (long)S::NPAIR * S::NCSP;
   R* out;
   if (NPB == 1)
-    out = (R*)a.out + (long)lev * S::NRSP;
+    out = (R*)a.out + mem * a.out_stride + (long)lev * S::NRSP;
   else
-    out = (R*)scratch + ((long)pb * a.nlev + lev) * S::NRSP;
+    out = (R*)scratch + ((long)pb * a.nmem * a.nlev + ml) * S::NRSP;
   for (int w = tid; w < S::NCSP; w += 256) {
     const int m = mw[w];
     const int n = m + (w - w0_of(T, m));
@@ -83,13 +84,17 @@ __global__ __launch_bounds__(256) void k_dirx(DirxArgs a, const void* tabv, void
   }
 }
 
+// sum of the latitude-pair blocks' partial sums in a fixed order; n reals
+// per block, per_mem of them per member (member stride out_stride in out)
 template <typename R>
-__global__ void k_sum_blocks(const R* scratch, R* out, int npb, long n) {
+__global__ void k_sum_blocks(const R* scratch, R* out, int npb, long n, long per_mem,
+                             long out_stride) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= n) return;
   R s = 0;
   for (int b = 0; b < npb; ++b) s += scratch[(long)b * n + i];
-  out[i] = s;
+  const long mem = i / per_mem;
+  out[mem * out_stride + (i - mem * per_mem)] = s;
 }
 
 template <int T, int MODE>
@@ -97,14 +102,15 @@ static int dirx_t(const Tables& tb, const DirxArgs& a, void* scratch, size_t sby
                   hipStream_t s) {
   using C = DirxCfg<T, MODE>;
   using R = RealX<MODE>;
-  if (a.nlev <= 0) return 0;
-  const long n = (long)a.nlev * TS<T>::NRSP;
+  if (a.nlev <= 0 || a.nmem <= 0) return 0;
+  const long per_mem = (long)a.nlev * TS<T>::NRSP;
+  const long n = (long)a.nmem * per_mem;
   if (C::NPB > 1 && (!scratch || sbytes < sizeof(R) * n * C::NPB)) return -4;
   const void* tv = (MODE == MODE_F64) ? (const void*)tb.filt64 : (const void*)tb.filt32;
-  k_dirx<T, MODE><<<(unsigned)(a.nlev * C::NPB), 256, 0, s>>>(a, tv, scratch);
+  k_dirx<T, MODE><<<(unsigned)((long)a.nmem * a.nlev * C::NPB), 256, 0, s>>>(a, tv, scratch);
   if (C::NPB > 1)
-    k_sum_blocks<R><<<(unsigned)((n + 255) / 256), 256, 0, s>>>((const R*)scratch, (R*)a.out,
-                                                                C::NPB, n);
+    k_sum_blocks<R><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(
+        (const R*)scratch, (R*)a.out, C::NPB, n, per_mem, a.out_stride);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

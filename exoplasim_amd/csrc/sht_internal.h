@@ -78,15 +78,17 @@ struct DirxTerm {
   int8_t table;
 };
 struct DirxArgs {
-  const void* in[3];  // Fourier inputs fc(2, NLON/2, NLAT, nlev)
-  void* out;          // spectral output (2*NCSP, nlev)
+  const void* in[3];  // Fourier inputs fc(2, NLON/2, NLAT, nlev) per member
+  void* out;          // spectral output (2*NCSP, nlev) per member
   int nin, nlev, nterm;
   DirxTerm terms[8];
+  int nmem;                   // members (>= 1)
+  long in_stride, out_stride; // member strides of the inputs / the output, in reals
 };
 
 int launch_dirx(const Tables& T, int mode, const DirxArgs& a, void* scratch, size_t sbytes,
                 hipStream_t s);
-size_t dirx_scratch_bytes(const Tables& T, int mode, int nlev);
+size_t dirx_scratch_bytes(const Tables& T, int mode, int nlev);  // nlev: member-levels
 
 int launch_inverse(const Tables& T, int mode, bool fused, const InvArgs& a, hipStream_t s);
 // the sp2fc (aq), sp2fcdmu (aj) and dv2uv (au) transforms of a round trip,

@@ -351,6 +351,78 @@ class Plan:
         check(lib().exo_sht_inverse_all(self._h, mode_id(mode), nmem, nlev,
                                         _ptr(sp_in), _ptr(gp), _stream()), "inverse_all")
 
+    # ---- Fourier-space forms (no fused FFT): Fourier arrays have the grid's
+    # shape (nmem, nfield, nlat, nlon) and hold fc(2, NLON/2, NLAT) per field
+    def sp2fc(self, mode, sp, fc, nfield, src0=0, dst0=0, table="qi", nmem=None):
+        """sp2fc (qi) or sp2fcdmu (qj) of field-levels [src0, src0+nfield)
+        into Fourier fields [dst0, ...) (exo_sht_sp2fc)."""
+        nmem = sp.shape[0] if nmem is None else nmem
+        if table not in ("qi", "qj"):
+            raise ExoError(f"sp2fc: table {table!r}, expected 'qi' or 'qj'")
+        dt = self.dtype_of(mode)
+        _fields("sp2fc sp", sp, dt, (self.nrsp,), nmem, (src0, nfield))
+        _fields("sp2fc fc", fc, dt, (self.nlat, self.nlon), nmem, (dst0, nfield))
+        check(lib().exo_sht_sp2fc(self._h, mode_id(mode), TABLES.index(table), nmem, _ptr(sp),
+                                  sp[0].numel(), src0, _ptr(fc), fc[0].numel(), dst0, nfield,
+                                  _stream()), "sp2fc")
+
+    def dv2uv(self, mode, sp, fc, nlev, vor0, div0, u0, v0, nmem=None):
+        nmem = sp.shape[0] if nmem is None else nmem
+        dt = self.dtype_of(mode)
+        _fields("dv2uv sp", sp, dt, (self.nrsp,), nmem, (vor0, nlev), (div0, nlev))
+        _fields("dv2uv fc", fc, dt, (self.nlat, self.nlon), nmem, (u0, nlev), (v0, nlev))
+        check(lib().exo_sht_dv2uv(self._h, mode_id(mode), nmem, _ptr(sp), sp[0].numel(), vor0,
+                                  div0, _ptr(fc), fc[0].numel(), u0, v0, nlev, _stream()),
+              "dv2uv")
+
+    def fc2sp(self, mode, fc, sp, nfield, src0=0, dst0=0, nmem=None):
+        nmem = fc.shape[0] if nmem is None else nmem
+        dt = self.dtype_of(mode)
+        _fields("fc2sp fc", fc, dt, (self.nlat, self.nlon), nmem, (src0, nfield))
+        _fields("fc2sp sp", sp, dt, (self.nrsp,), nmem, (dst0, nfield))
+        check(lib().exo_sht_fc2sp(self._h, mode_id(mode), nmem, _ptr(fc), fc[0].numel(), src0,
+                                  _ptr(sp), sp[0].numel(), dst0, nfield, _stream()), "fc2sp")
+
+    def fft(self, mode, a, direction):
+        """In-place gp2fc (direction -1) or fc2gp (+1) of every row of a
+        (any shape whose last dimension is nlon)."""
+        if a.dtype != self.dtype_of(mode) or a.dim() < 1 or a.shape[-1] != self.nlon:
+            raise ExoError(f"fft: {a.dtype} {tuple(a.shape)}, expected rows of {self.nlon} "
+                           f"{self.dtype_of(mode)}")
+        check(lib().exo_sht_fft(self._h, mode_id(mode), int(direction), _ptr(a),
+                                a.numel() // self.nlon, _stream()), "fft")
+
+    def _multi(self, what, mode, nlev, fourier, spectral):
+        dt = self.dtype_of(mode)
+        nmem = fourier[0].shape[0]
+        for t in fourier:
+            _fields(f"{what} Fourier input", t, dt, (self.nlat, self.nlon), nmem, (0, nlev))
+            if t.shape != fourier[0].shape:
+                raise ExoError(f"{what}: Fourier inputs of different shapes")
+        for t in spectral:
+            _fields(f"{what} spectral output", t, dt, (self.nrsp,), nmem, (0, nlev))
+            if t.shape != spectral[0].shape:
+                raise ExoError(f"{what}: spectral outputs of different shapes")
+        return nmem, fourier[0][0].numel(), spectral[0][0].numel()
+
+    def uv2dv(self, mode, pu, pv, pd, pz, nlev):
+        """uv2dv of Fourier u, v (nmem, >=nlev, nlat, nlon) into spectral D,
+        zeta (nmem, >=nlev, nrsp) (exo_sht_uv2dv)."""
+        nmem, fs, ss = self._multi("uv2dv", mode, nlev, (pu, pv), (pd, pz))
+        check(lib().exo_sht_uv2dv(self._h, mode_id(mode), nmem, nlev, _ptr(pu), _ptr(pv), fs,
+                                  _ptr(pd), _ptr(pz), ss, _stream()), "uv2dv")
+
+    def qtend(self, mode, q, qn, uq, vq, nlev):
+        nmem, fs, ss = self._multi("qtend", mode, nlev, (qn, uq, vq), (q,))
+        check(lib().exo_sht_qtend(self._h, mode_id(mode), nmem, nlev, _ptr(q), ss, _ptr(qn),
+                                  _ptr(uq), _ptr(vq), fs, _stream()), "qtend")
+
+    def mktend(self, mode, d, t, z, tn, fu, fv, ke, ut, vt, nlev):
+        nmem, fs, ss = self._multi("mktend", mode, nlev, (tn, fu, fv, ke, ut, vt), (d, t, z))
+        check(lib().exo_sht_mktend(self._h, mode_id(mode), nmem, nlev, _ptr(d), _ptr(t),
+                                   _ptr(z), ss, _ptr(tn), _ptr(fu), _ptr(fv), _ptr(ke),
+                                   _ptr(ut), _ptr(vt), fs, _stream()), "mktend")
+
     def set_tiles(self, mode, inv_var=0, uv_var=0, fwd_var=0):
         check(lib().exo_sht_set_tiles(self._h, mode_id(mode), inv_var, uv_var, fwd_var),
               "set_tiles")

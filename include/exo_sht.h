@@ -161,6 +161,44 @@ int exo_sht_roundtrip(exo_sht_plan_t plan, int mode, int nmem, int nlev, const v
 int exo_sht_inverse_all(exo_sht_plan_t plan, int mode, int nmem, int nlev, const void *sp_in,
                         void *gp, void *stream);
 
+/* Device-resident forms of the legmod / fftmod routines that work in
+ * Fourier space, on device arrays of nmem members (member strides in reals),
+ * asynchronous on stream; element type double for EXO_F64 else float.
+ * Fourier arrays are fc(2, NLON/2, NLAT) per field-level, the same memory as
+ * a grid field.
+ *   exo_sht_sp2fc   sp2fc / sp2fcdmu (src/legmod.f90:230,262), table EXO_QI
+ *                   or EXO_QJ: spectral field-levels [src0, src0+nfield) ->
+ *                   Fourier fields [dst0, ...)
+ *   exo_sht_dv2uv   dv2uv (src/legmod.f90:312): vorticity levels vor0..,
+ *                   divergence levels div0.. -> Fourier u*cos u0.., v*cos v0..
+ *   exo_sht_fc2sp   fc2sp (src/legmod.f90:178), folded branch
+ *   exo_sht_fft     gp2fc (dir -1) / fc2gp (dir +1) (src/fftmod.f90:29,64)
+ *                   in place over lot rows of the plan's NLON
+ *   exo_sht_uv2dv   uv2dv (src/legmod.f90:356): Fourier u, v (nlev levels per
+ *                   member, member stride f_stride) -> spectral D, zeta
+ *                   (member stride s_stride)
+ *   exo_sht_qtend   qtend (src/legmod.f90:433)
+ *   exo_sht_mktend  mktend (src/legmod.f90:505); the model passes fu <- gvz,
+ *                   fv <- guz (src/plasim.f90:2478)
+ * The multi-input direct transforms (uv2dv, qtend, mktend) sum on the vector
+ * ALUs in FP64 (EXO_F64) or FP32 (every other mode). */
+int exo_sht_sp2fc(exo_sht_plan_t plan, int mode, int table, int nmem, const void *sp,
+                  long sp_stride, int src0, void *fc, long fc_stride, int dst0, int nfield,
+                  void *stream);
+int exo_sht_dv2uv(exo_sht_plan_t plan, int mode, int nmem, const void *sp, long sp_stride,
+                  int vor0, int div0, void *fc, long fc_stride, int u0, int v0, int nlev,
+                  void *stream);
+int exo_sht_fc2sp(exo_sht_plan_t plan, int mode, int nmem, const void *fc, long fc_stride,
+                  int src0, void *sp, long sp_stride, int dst0, int nfield, void *stream);
+int exo_sht_fft(exo_sht_plan_t plan, int mode, int dir, void *a, long lot, void *stream);
+int exo_sht_uv2dv(exo_sht_plan_t plan, int mode, int nmem, int nlev, const void *pu,
+                  const void *pv, long f_stride, void *pd, void *pz, long s_stride, void *stream);
+int exo_sht_qtend(exo_sht_plan_t plan, int mode, int nmem, int nlev, void *q, long s_stride,
+                  const void *qn, const void *uq, const void *vq, long f_stride, void *stream);
+int exo_sht_mktend(exo_sht_plan_t plan, int mode, int nmem, int nlev, void *d, void *t, void *z,
+                   long s_stride, const void *tn, const void *fu, const void *fv, const void *ke,
+                   const void *ut, const void *vt, long f_stride, void *stream);
+
 /* Tile variants of the fused kernels: vars[0] the inverse sp2fc/sp2fcdmu
  * kernel and the one-launch inverse of exo_sht_inverse_all /
  * exo_sht_roundtrip, vars[1] the dv2uv kernel of exo_sht_inverse_uv,

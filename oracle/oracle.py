@@ -12,6 +12,7 @@ fastest); see exo_oracle.h.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 from ctypes import POINTER, byref, c_double, c_int, c_long
 
@@ -28,6 +29,15 @@ TABLES = ("qi", "qj", "qc", "qe", "qm", "qq", "qu", "qv")
 PLAVOR = 1.63299310207
 # kappa = R / cp of the Earth set-up (src/p_earth.f90:41-58)
 AKAP_EARTH = 0.286
+
+
+def _libm():
+    """the C library's single-precision exp, which the reference's
+    default-real Fortran calls"""
+    L = ctypes.CDLL("libm.so.6")
+    L.expf.restype = ctypes.c_float
+    L.expf.argtypes = [ctypes.c_float]
+    return L
 
 
 def _p(a: np.ndarray):
@@ -355,10 +365,76 @@ class RefT21:
 
 class RefPyfft:
     """The reference's resolution-parameterised transform copies
-    (exoplasim/pyfft.f90), FP64, no filter (nfs=0)."""
+    (exoplasim/pyfft.f90): kind "r8" is the -fdefault-real-8 build (every
+    literal in double precision), kind "r4" the default-real build the
+    post-processor's f2py step makes (exoplasim/__init__.py:84-87).  The
+    wrappers below take and return C-ordered numpy arrays: spectral
+    (nlev, nrsp), grid (nlev, nlat, nlon), Fourier (nlev, nlat, nlon)."""
 
-    def __init__(self):
-        self.lib = ctypes.CDLL(_ref_path("libref_pyfft.so"))
+    def __init__(self, kind: str = "r8"):
+        name = {"r8": "libref_pyfft.so", "r4": "libref_pyfft_r4.so"}[kind]
+        self.lib = ctypes.CDLL(_ref_path(name))
+
+    @staticmethod
+    def _i(*v):
+        return [byref(c_int(int(x))) for x in v]
+
+    def sp2gp(self, ntru, nlat, sp, nfs=0):
+        sp = np.ascontiguousarray(sp, dtype=np.float64)
+        nlev = sp.shape[0]
+        gp = np.zeros((nlev, nlat, 2 * nlat))
+        self.lib.sp2gp_(_p(sp), _p(gp), *self._i(nlat, 2 * nlat, ntru, nlev, nfs))
+        return gp
+
+    def gp2sp(self, ntru, gp, nfs=0):
+        gp = np.ascontiguousarray(gp, dtype=np.float64)
+        nlev, nlat, nlon = gp.shape
+        sp = np.zeros((nlev, (ntru + 1) * (ntru + 2)))
+        self.lib.gp2sp_(_p(gp), _p(sp), *self._i(nlat, nlon, ntru, nlev, nfs))
+        return sp
+
+    def sp3fc(self, ntru, nlat, sp, nfs=0):
+        sp = np.ascontiguousarray(sp, dtype=np.float64)
+        nlev = sp.shape[0]
+        fc = np.zeros((nlev, nlat, 2 * nlat))
+        self.lib.sp3fc_(_p(sp), _p(fc), *self._i(nlat, 2 * nlat, ntru, nlev, nfs))
+        return fc
+
+    def gp3fc(self, gp):
+        gp = np.ascontiguousarray(gp, dtype=np.float64)
+        nlev, nlat, nlon = gp.shape
+        fc = np.zeros_like(gp)
+        self.lib.gp3fc_(_p(gp), _p(fc), *self._i(nlon, nlat, nlev))
+        return fc
+
+    def fc3gp(self, fc):
+        fc = np.ascontiguousarray(fc, dtype=np.float64)
+        nlev, nlat, nlon = fc.shape
+        gp = np.zeros_like(fc)
+        self.lib.fc3gp_(_p(fc), _p(gp), *self._i(nlon, nlat, nlev))
+        return gp
+
+    def spvgp(self, ntru, sd, sz, rdcostheta, nfs=0):
+        sd = np.ascontiguousarray(sd, dtype=np.float64)
+        sz = np.ascontiguousarray(sz, dtype=np.float64)
+        rd = np.ascontiguousarray(rdcostheta, dtype=np.float64)
+        nlev, nlat = sd.shape[0], rd.size
+        gu = np.zeros((nlev, nlat, 2 * nlat))
+        gv = np.zeros_like(gu)
+        self.lib.spvgp_(_p(sd), _p(sz), _p(rd), _p(gu), _p(gv),
+                        *self._i(nlat, 2 * nlat, ntru, nlev, nfs))
+        return gu, gv
+
+    def gpvsp(self, ntru, gu, gv, costhetadr, nfs=0):
+        gu = np.ascontiguousarray(gu, dtype=np.float64)
+        gv = np.ascontiguousarray(gv, dtype=np.float64)
+        cd = np.ascontiguousarray(costhetadr, dtype=np.float64)
+        nlev, nlat, nlon = gu.shape
+        sd = np.zeros((nlev, (ntru + 1) * (ntru + 2)))
+        sz = np.zeros_like(sd)
+        self.lib.gpvsp_(_p(gu), _p(gv), _p(cd), _p(sd), _p(sz),
+                        *self._i(nlat, nlon, ntru, nlev, nfs))
+        return sd, sz
 
     def legini(self, ntru, nlat):
         nlon = 2 * nlat
@@ -374,6 +450,15 @@ class RefPyfft:
             tab[i] = bufs[i]
         return tab
 
+    def sfilt(self, ntru, nlat, nfs):
+        """the physics filter legini returns (pyfft.f90:150-153)"""
+        ncsp = (ntru + 1) * (ntru + 2) // 2
+        bufs = [np.zeros((nlat, ncsp)) for _ in range(8)]
+        sf = np.zeros(ntru + 1)
+        self.lib.legini_(byref(c_int(nlat)), byref(c_int(2 * nlat)), byref(c_int(ntru)),
+                         byref(c_int(1)), *[_p(b) for b in bufs], _p(sf), byref(c_int(nfs)))
+        return sf
+
     def sp2fc(self, ntru, nlat, sp):
         fc = np.zeros((nlat, 2 * nlat))
         self.lib.sp2fc_(_p(np.ascontiguousarray(sp)), _p(fc), byref(c_int(nlat)),
@@ -388,6 +473,98 @@ class RefPyfft:
                         byref(c_int(2 * nlat)), byref(c_int(ntru)), byref(c_int(1)),
                         byref(c_int(0)))
         return sp
+
+
+class PyfftOracle:
+    """The post-processor's transform wrappers (exoplasim/pyfft.f90:
+    sp2gp :971-1005, spvgp :1012-1069, gpvsp :1076-1134, gp2sp :1141-1174,
+    sp3fc :344-370, fc3gp :905-930, gp3fc :938-963) restated over the
+    oracle's legmod/fftmod transforms: pyfft's legini has the same tables as
+    legmod (pyfft.f90:73-146), dv2uv has no planetary vorticity
+    (pyfft.f90:527-545), one filter serves both directions, and fc2gp ends
+    with a multiplication by sqrt 2 (pyfft.f90:891).
+
+    kind "r4" evaluates the default-real pieces as the post-processor's
+    default-real build does: the sqrt 2 literal and the filter formula
+    (pyfft.f90:150-153) in single precision.  kind "r8" evaluates them in
+    double precision, as the -fdefault-real-8 build of the same file does.
+    Arrays are C-ordered as RefPyfft's."""
+
+    def __init__(self, oracle: "Oracle", kind: str = "r4"):
+        self.o = oracle
+        self.kind = kind
+        self.sqrt2 = (float(np.float32(1.4142135623730951)) if kind == "r4"
+                      else 1.4142135623730951)
+        self._tabs = {}
+
+    def sfilt(self, ntru, nfs):
+        """pyfft.f90:150-153, value of n = 1 .. NTRU+1 at index n-1.  The
+        power x**8 is evaluated as seven successive products and exp by the C
+        library, as the reference compiled here (amdflang) evaluates them;
+        in the default-real build a compiler that expands the power by
+        squaring instead moves the filter tail by up to 3e-6 relative."""
+        if nfs == 0:
+            return np.ones(ntru + 1)
+        if self.kind == "r8":
+            x = np.arange(1, ntru + 2, dtype=np.float64) / ntru
+            p = x.copy()
+            for _ in range(7):
+                p = p * x
+            return np.array([math.exp(-8.0 * v) for v in p])
+        x = np.arange(1, ntru + 2, dtype=np.float32) / np.float32(ntru)
+        p = x.copy()
+        for _ in range(7):
+            p = p * x
+        expf = _libm().expf
+        return np.array([expf(float(np.float32(-8.0) * v)) for v in p], dtype=np.float64)
+
+    def _t(self, ntru, nlat):
+        if (ntru, nlat) not in self._tabs:
+            self._tabs[(ntru, nlat)] = self.o.legini(ntru, nlat)
+        return self._tabs[(ntru, nlat)]
+
+    def sp2gp(self, ntru, nlat, sp, nfs=0):
+        sp = np.atleast_2d(sp)
+        t, f = self._t(ntru, nlat), self.sfilt(ntru, nfs)
+        fc = np.stack([self.o.sp2fc(ntru, nlat, 2 * nlat, t["qi"], f, x) for x in sp])
+        return self.o.fc2gp(fc) * self.sqrt2
+
+    def gp2sp(self, ntru, gp, nfs=0):
+        gp = np.asarray(gp, dtype=np.float64)
+        nlat = gp.shape[1]
+        t, f = self._t(ntru, nlat), self.sfilt(ntru, nfs)
+        fc = self.o.gp2fc(gp / self.sqrt2)
+        return np.stack([self.o.fc2sp(ntru, nlat, 2 * nlat, t["qc"], f, x) for x in fc])
+
+    def sp3fc(self, ntru, nlat, sp, nfs=0):
+        sp = np.atleast_2d(sp)
+        t, f = self._t(ntru, nlat), self.sfilt(ntru, nfs)
+        return np.stack([self.o.sp2fc(ntru, nlat, 2 * nlat, t["qi"], f, x) for x in sp])
+
+    def gp3fc(self, gp):
+        return self.o.gp2fc(gp)
+
+    def fc3gp(self, fc):
+        return self.o.fc2gp(fc) * self.sqrt2
+
+    def spvgp(self, ntru, sd, sz, rdcostheta, nfs=0):
+        sd, sz = np.atleast_2d(sd), np.atleast_2d(sz)
+        rd = np.asarray(rdcostheta, dtype=np.float64)
+        nlat = rd.size
+        t, f = self._t(ntru, nlat), self.sfilt(ntru, nfs)
+        pu, pv = self.o.dv2uv(ntru, nlat, 2 * nlat, sd.shape[0], t["qu"], t["qv"], f, 0.0,
+                              sd, sz)
+        gu = (self.o.fc2gp(pu) * self.sqrt2) * rd[None, :, None]
+        gv = (self.o.fc2gp(pv) * self.sqrt2) * rd[None, :, None]
+        return gu, gv
+
+    def gpvsp(self, ntru, gu, gv, costhetadr, nfs=0):
+        cd = np.asarray(costhetadr, dtype=np.float64)
+        nlat = cd.size
+        t, f = self._t(ntru, nlat), self.sfilt(ntru, nfs)
+        fu = self.o.gp2fc((np.asarray(gu) * cd[None, :, None]) / self.sqrt2)
+        fv = self.o.gp2fc((np.asarray(gv) * cd[None, :, None]) / self.sqrt2)
+        return self.o.uv2dv(ntru, nlat, 2 * nlat, fu.shape[0], t["qe"], t["qm"], f, fu, fv)
 
 
 class RefPlasim:

@@ -83,6 +83,54 @@ def main():
     filter_golden(r, sp, nl)
     plasim_golden()
     lwr_golden()
+    pyfft_golden()
+
+
+def pyfft_inputs(ntru, nlat, nlev):
+    """inputs of the post-processor wrapper pins: spectral levels, grid
+    levels and the 1/cos and cos latitude factors pyburn passes
+    (exoplasim/pyburn.py:1040-1067, 1150-1172 use radius / cos and cos /
+    radius; unit radius here)"""
+    from oracle import Oracle
+
+    sid, _ = Oracle().inigau(nlat)
+    coslat = np.sqrt(1.0 - sid * sid)
+    sd = spectral_input(ntru, nlev, 1.7)
+    sz = spectral_input(ntru, nlev, 2.3)
+    gu = golden_input((nlev, nlat, 2 * nlat), 3.1)
+    gv = golden_input((nlev, nlat, 2 * nlat), 3.7)
+    return sd, sz, gu, gv, 1.0 / coslat, coslat
+
+
+def pyfft_golden():
+    """sp2gp, gp2sp, sp3fc, gp3fc, fc3gp, spvgp, gpvsp of exoplasim/pyfft.f90
+    in its default-real build (the post-processor's own) at T21, one level,
+    with and without the physics filter; sp2gp and gp2sp also from the
+    -fdefault-real-8 build with the filter."""
+    from oracle import RefPyfft
+
+    out = {}
+    ntru, nlat, nlev = 21, 32, 1
+    sd, sz, gu, gv, rd, cd = pyfft_inputs(ntru, nlat, nlev)
+    for kind in ("r4", "r8"):
+        p = RefPyfft(kind)
+        for nfs in ((0, 1) if kind == "r4" else (1,)):
+            k = f"{kind}_nfs{nfs}_"
+            out[k + "sp2gp"] = p.sp2gp(ntru, nlat, sd, nfs)
+            out[k + "gp2sp"] = p.gp2sp(ntru, gu, nfs)
+            if kind == "r8":
+                continue
+            out[k + "sp3fc"] = p.sp3fc(ntru, nlat, sd, nfs)
+            out[k + "spvgp_u"], out[k + "spvgp_v"] = p.spvgp(ntru, sd, sz, rd, nfs)
+            out[k + "gpvsp_d"], out[k + "gpvsp_z"] = p.gpvsp(ntru, gu, gv, cd, nfs)
+        if kind == "r4":
+            out["r4_gp3fc"] = p.gp3fc(gu)
+            out["r4_fc3gp"] = p.fc3gp(gu)
+            for t, nl in ((21, 32), (42, 64), (63, 96), (127, 192)):
+                out[f"r4_sfilt_t{t}"] = p.sfilt(t, nl, 1)
+    path = os.path.join(HERE, "reference_pyfft_t21.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes")
 
 
 # physics filters of legini (src/legmod.f90:123-155): nfilter 1..4 with both
@@ -226,5 +274,7 @@ if __name__ == "__main__":
         lwr_golden()
     elif sys.argv[1:] == ["plasim"]:
         plasim_golden()
+    elif sys.argv[1:] == ["pyfft"]:
+        pyfft_golden()
     else:
         main()
