@@ -51,6 +51,8 @@ PROTOTYPES = {
     "exo_sht_roundtrip": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p,
                                   c_void_p, c_void_p]),
     "exo_sht_inverse_all": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "exo_sht_roundtrip_noalloc": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p,
+                                          c_void_p, c_void_p]),
     "exo_sht_sp2fc": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_long, c_int,
                               c_void_p, c_long, c_int, c_int, c_void_p]),
     "exo_sht_dv2uv": (c_int, [c_void_p, c_int, c_int, c_void_p, c_long, c_int, c_int,

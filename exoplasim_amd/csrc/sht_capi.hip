@@ -150,6 +150,39 @@ int plan_grow(void** p, size_t* have, size_t need) {
 }
 
 // ---------------------------------------------------------------------------
+// the no-allocation round trip's scratch
+// ---------------------------------------------------------------------------
+// Device memory of exo_sht_roundtrip_noalloc: per chunk of members, the
+// pre-split operand of the split-BF16 inverse, then the partial direct sums
+// of truncations whose direct transform runs in several latitude-pair
+// blocks.  Each device holds its own copy of the array, from the loading of
+// this library's code on it until the process ends.
+static __device__ __align__(256) char scratch[EXO_NOALLOC_SCRATCH_BYTES];
+
+// The array's address on each device, taken once per device with that
+// device current.  In host code the name `scratch` is only a host-side
+// stand-in for the symbol; kernels and copies get these addresses.
+static constexpr int MAX_DEVICES = 64;
+static char* g_scratch_dev[MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+static int scratch_address(char** out) {
+  *out = nullptr;
+  int dev = -1;
+  HIPCK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return EXO_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    HIPCK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return EXO_ERR_HIP;
+    g_scratch_dev[dev] = (char*)ptr;
+  }
+  *out = g_scratch_dev[dev];
+  return EXO_OK;
+}
+
+// ---------------------------------------------------------------------------
 // plans
 // ---------------------------------------------------------------------------
 extern "C" int exo_sht_plan_create(exo_sht_plan_t* plan, int ntru, const double* sid,
@@ -159,6 +192,9 @@ extern "C" int exo_sht_plan_create(exo_sht_plan_t* plan, int ntru, const double*
   *plan = nullptr;
   exo_sht_plan_s* p = new exo_sht_plan_s();
   int rc = build_tables(p->tb, ntru, sid, gwd, skgpsp, skspgp, plavor);
+  // the no-allocation scratch of this device (the plan's tables live here);
+  // a failure is reported by exo_sht_roundtrip_noalloc, which needs it
+  if (rc == EXO_OK) (void)scratch_address(&p->na_scratch);
   if (rc != EXO_OK) {
     free_tables(p->tb);
     delete p;
@@ -360,6 +396,68 @@ extern "C" int exo_sht_roundtrip(exo_sht_plan_t p, int mode, int nmem, int nlev,
   const int rc = inverse_all_impl(p, mode, nmem, nlev, sp_in, gp, s);
   if (rc) return rc;
   return direct_impl(p, mode, true, nmem, gp, gps, 0, sp_out, sps, 0, nfl, s);
+}
+
+// Bytes of scratch one chunk of c members needs: the pre-split operand
+// (split-BF16) at offset 0, the partial direct sums from pk_off(c) on.
+static size_t na_pk_off(const Tables& t, int mode, int nfl, int c) {
+  const size_t pk = (mode == MODE_BF16X3) ? pack_bytes(t, c, nfl) : 0;
+  return (pk + 255) & ~(size_t)255;
+}
+static size_t na_bytes(const Tables& t, int mode, int nfl, int c) {
+  return na_pk_off(t, mode, nfl, c) + forward_scratch_bytes(t, mode, c, nfl);
+}
+
+extern "C" int exo_sht_roundtrip_noalloc(exo_sht_plan_t p, int mode, int nmem, int nlev,
+                                         const void* sp_in, void* gp, void* sp_out,
+                                         void* stream) {
+  if (!p || !mode_ok(mode) || nmem < 0 || nlev < 1) return EXO_ERR_ARG;
+  if (nmem > 0 && (!sp_in || !gp || !sp_out)) return EXO_ERR_ARG;
+  std::lock_guard<std::mutex> lk(p->mu);
+  const Tables& t = p->tb;
+  if (!p->na_scratch) return EXO_ERR_HIP;
+  // the plan's device must be the current one: its scratch address is
+  // that device's
+  char* dev_scratch = nullptr;
+  const int rc0 = scratch_address(&dev_scratch);
+  if (rc0 != EXO_OK) return rc0;
+  if (dev_scratch != p->na_scratch) return EXO_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const int nfl = 3 * nlev + 1;
+  const int ngp = nfl + 1 + 2 * nlev;
+  const long sps = (long)nfl * 2 * t.ncsp;
+  const long gps = (long)ngp * t.nlat * t.nlon;
+  // the largest chunk of members whose scratch fits the array
+  if (na_bytes(t, mode, nfl, 1) > EXO_NOALLOC_SCRATCH_BYTES) return EXO_ERR_SCRATCH;
+  int chunk = 1;
+  while (chunk < nmem && na_bytes(t, mode, nfl, 2 * chunk) <= EXO_NOALLOC_SCRATCH_BYTES)
+    chunk *= 2;
+  {
+    int lo = chunk, hi = 2 * chunk;  // na_bytes(lo) fits, na_bytes(hi) does not (or hi > nmem)
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) / 2;
+      if (na_bytes(t, mode, nfl, mid) <= EXO_NOALLOC_SCRATCH_BYTES)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    chunk = lo < nmem ? lo : (nmem > 0 ? nmem : 1);
+  }
+  const size_t es = (mode == MODE_F64) ? 8 : 4;
+  for (int m0 = 0; m0 < nmem; m0 += chunk) {
+    const int c = (nmem - m0 < chunk) ? nmem - m0 : chunk;
+    const char* spi = (const char*)sp_in + es * sps * m0;
+    char* g = (char*)gp + es * gps * m0;
+    char* spo = (char*)sp_out + es * sps * m0;
+    char* pk = p->na_scratch;
+    char* fwd = p->na_scratch + na_pk_off(t, mode, nfl, c);
+    const size_t fwd_bytes = EXO_NOALLOC_SCRATCH_BYTES - na_pk_off(t, mode, nfl, c);
+    int rc = inverse_all_impl(p, mode, c, nlev, spi, g, s, pk);
+    if (rc) return rc;
+    rc = direct_impl(p, mode, true, c, g, gps, 0, spo, sps, 0, nfl, s, fwd, fwd_bytes);
+    if (rc) return rc;
+  }
+  return EXO_OK;
 }
 
 extern "C" int exo_sht_set_tiles(exo_sht_plan_t p, int mode, int inv_var, int uv_var,

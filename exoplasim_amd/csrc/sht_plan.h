@@ -13,6 +13,9 @@ struct exo_sht_plan_s {
   void* pk = nullptr;       // pre-split spectral operand of the split-BF16 inverse
   size_t pk_bytes = 0;
   int nlev = 0;
+  // device address of the file-scope `scratch` array on the device the
+  // plan was created on (exo_sht_roundtrip_noalloc)
+  char* na_scratch = nullptr;
   // tile variants per precision mode, chosen by exo_sht_autotune
   int inv_var[4] = {0, 0, 0, 0}, uv_var[4] = {0, 0, 0, 0}, fwd_var[4] = {0, 0, 0, 0};
   std::mutex mu;

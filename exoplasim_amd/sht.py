@@ -345,6 +345,14 @@ class Plan:
                                       _ptr(sp_in), _ptr(gp), _ptr(sp_out), _stream()),
               "roundtrip")
 
+    def roundtrip_noalloc(self, mode, sp_in, gp, sp_out, nlev):
+        """The same round trip with no device allocation: the scratch is the
+        library's fixed per-GPU array (exo_sht_roundtrip_noalloc)."""
+        nmem = self._roundtrip_buffers("roundtrip_noalloc", mode, nlev, sp_in, gp, sp_out)
+        check(lib().exo_sht_roundtrip_noalloc(self._h, mode_id(mode), nmem, nlev,
+                                              _ptr(sp_in), _ptr(gp), _ptr(sp_out), _stream()),
+              "roundtrip_noalloc")
+
     def inverse_all(self, mode, sp_in, gp, nlev):
         """The inverse half of a round trip in one launch (exo_sht_inverse_all)."""
         nmem = self._roundtrip_buffers("inverse_all", mode, nlev, sp_in, gp)

@@ -161,6 +161,23 @@ int exo_sht_roundtrip(exo_sht_plan_t plan, int mode, int nmem, int nlev, const v
 int exo_sht_inverse_all(exo_sht_plan_t plan, int mode, int nmem, int nlev, const void *sp_in,
                         void *gp, void *stream);
 
+/* No counterpart in the reference: exo_sht_roundtrip for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, layouts and
+ * results as exo_sht_roundtrip; it calls no hipMalloc or hipFree.  Its
+ * scratch (the pre-split operand of EXO_BF16X3 and the partial direct sums
+ * of T63 in FP64 and of T127) is a file-scope device array of
+ * EXO_NOALLOC_SCRATCH_BYTES per GPU, held from the library's loading on
+ * that GPU until the process ends; members run in chunks whose scratch
+ * fits it.  EXO_ERR_SCRATCH when one member's scratch exceeds it (T127
+ * L40 in EXO_F64 needs 96 MB per member).  The plan must be used on the
+ * device it was created on.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other plans, other
+ * threads) race and must be ordered by the caller. */
+#define EXO_NOALLOC_SCRATCH_BYTES (64u << 20)
+int exo_sht_roundtrip_noalloc(exo_sht_plan_t plan, int mode, int nmem, int nlev,
+                              const void *sp_in, void *gp, void *sp_out, void *stream);
+
 /* Device-resident forms of the legmod / fftmod routines that work in
  * Fourier space, on device arrays of nmem members (member strides in reals),
  * asynchronous on stream; element type double for EXO_F64 else float.
