@@ -19,7 +19,8 @@ OBJDIR = os.path.join(HERE, "_build")
 SOURCES = ["sht_tables.hip", "sht_transforms.hip", "sht_transforms_m0.hip",
            "sht_transforms_m1.hip", "sht_transforms_m2.hip", "sht_transforms_m3.hip",
            "sht_direct_terms.hip", "sht_capi.hip", "sht_ensemble.hip", "si_solve.hip",
-           "rad_lwr.hip", "rad_mlp.hip"]
+           "dyn_calcgp.hip", "sht_multiproc.hip", "sht_output.hip", "rad_lwr.hip",
+           "rad_mlp.hip"]
 HIPFLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions are kept out of every kernel

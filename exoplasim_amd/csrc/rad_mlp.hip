@@ -271,6 +271,7 @@ inline unsigned nb(long n, int b) { return (unsigned)((n + b - 1) / b); }
 // as one 8-byte store of the row-major activations.
 // ---------------------------------------------------------------------------
 constexpr int EM_NLEV = 20, EM_KIN = 64, EM_NH = 256, EM_NOUT = 21, EM_NOUTP = 32;
+constexpr int EM_NIN = 2 * EM_NLEV + 3;  // features: T, ln q, T_s, ln p_s, ln CO2
 constexpr int EM_MT = 32;             // columns per block
 constexpr int EM_NT = 512;            // threads per block (8 waves)
 constexpr int EM_LDH = EM_NH + 8;     // padded LDS row (bf16) of a hidden layer
@@ -330,14 +331,24 @@ __device__ __forceinline__ void em_hidden(const uint16_t* hin, const V8 (&wa)[K 
   }
 }
 
+// GUARD: also flag (outside[col] = 1) every column with a feature outside
+// [flo, fhi] (or not a number): the columns the network was not trained on
+template <bool GUARD>
 __global__ __launch_bounds__(EM_NT) void k_emulate(exo_lw_net net, int ncol, const float* dt,
                                                    const float* dq, const float* dp,
-                                                   const float* co2, float* heat, float* olr) {
+                                                   const float* co2, float* heat, float* olr,
+                                                   const float* flo, const float* fhi,
+                                                   int* outside) {
   __shared__ __align__(16) uint16_t sx[EM_MT * EM_LDX];
   __shared__ __align__(16) uint16_t sh0[EM_MT * EM_LDH];
   __shared__ __align__(16) uint16_t sh1[EM_MT * EM_LDH];
+  __shared__ int sflag[EM_MT];
   const int tid = threadIdx.x;
   const int col0 = blockIdx.x * EM_MT;
+  if (GUARD) {
+    if (tid < EM_MT) sflag[tid] = 0;
+    __syncthreads();
+  }
   // the weights of the first two layers are on their way while the features
   // are computed; the third layer's while the second runs
   V8 w1[EM_KIN / 32][2], w2[EM_NH / 32][2], w3[EM_NH / 32][2];
@@ -359,8 +370,10 @@ __global__ __launch_bounds__(EM_NT) void k_emulate(exo_lw_net net, int ncol, con
     else if (f == 2 * EM_NLEV + 2)
       v = (logf(co2[col]) - 5.75f) / 3.3f;
     sx[m * EM_LDX + f] = f2bf(v);
+    if (GUARD && f < EM_NIN && !(v >= flo[f] && v <= fhi[f])) atomicOr(&sflag[m], 1);
   }
   __syncthreads();
+  if (GUARD && tid < EM_MT && col0 + tid < ncol) outside[col0 + tid] = sflag[tid];
   em_hidden<EM_KIN, EM_LDX>(sx, w1, net.b[0], sh0);
   __syncthreads();
   em_load<EM_NH>((const uint16_t*)net.w[2], w3);
@@ -461,8 +474,24 @@ extern "C" int exo_lw_emulate(const exo_lw_net* net, int ncol, const float* dt, 
     return EXO_RAD_ERR_ARG;
   for (int l = 0; l < 4; ++l)
     if (!net->w[l] || !net->b[l]) return EXO_RAD_ERR_ARG;
-  k_emulate<<<nb(ncol, EM_MT), EM_NT, 0, (hipStream_t)stream>>>(*net, ncol, dt, dq, dp, dqco2,
-                                                                 heat, olr);
+  k_emulate<false><<<nb(ncol, EM_MT), EM_NT, 0, (hipStream_t)stream>>>(
+      *net, ncol, dt, dq, dp, dqco2, heat, olr, nullptr, nullptr, nullptr);
+  return hipGetLastError() == hipSuccess ? EXO_RAD_OK : EXO_RAD_ERR_HIP;
+}
+
+extern "C" int exo_lw_emulate_guarded(const exo_lw_net* net, const float* flo, const float* fhi,
+                                      int ncol, const float* dt, const float* dq,
+                                      const float* dp, const float* dqco2, float* heat,
+                                      float* olr, int* outside, void* stream) {
+  if (!net || ncol < 0 || net->nlev != EM_NLEV) return EXO_RAD_ERR_ARG;
+  if (ncol == 0) return EXO_RAD_OK;
+  if (!dt || !dq || !dp || !dqco2 || !heat || !olr || !net->ymean || !net->ystd || !flo ||
+      !fhi || !outside)
+    return EXO_RAD_ERR_ARG;
+  for (int l = 0; l < 4; ++l)
+    if (!net->w[l] || !net->b[l]) return EXO_RAD_ERR_ARG;
+  k_emulate<true><<<nb(ncol, EM_MT), EM_NT, 0, (hipStream_t)stream>>>(
+      *net, ncol, dt, dq, dp, dqco2, heat, olr, flo, fhi, outside);
   return hipGetLastError() == hipSuccess ? EXO_RAD_OK : EXO_RAD_ERR_HIP;
 }
 

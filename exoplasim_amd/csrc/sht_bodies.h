@@ -358,7 +358,15 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
           const int pl = 2 * q + h;
           const float sy = out[2 * h], an = out[2 * h + 1];
           // component ri of the northern and southern coefficient
-          const float nv = sy + an, sv = sy - an;
+          float nv = sy + an, sv = sy - an;
+          if (KIND == INV_SCALAR && a.dlam) {
+            // d/dlambda: (x + i y) i m = -m y + i m x, the other component
+            // from the partner column (lane ^ 1)
+            const float fm = (float)m;
+            const float np = quad_perm<0xB1>(nv), spn = quad_perm<0xB1>(sv);
+            nv = ri ? np * fm : -np * fm;
+            sv = ri ? spn * fm : -spn * fm;
+          }
           if constexpr (FUSED) {
             // the partner column (lane ^ 1) holds the other component: sp
             // is s.y for ri = 0 and s.x for ri = 1
@@ -456,7 +464,13 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
           }
         }
       }
-      put(pl, fl, m, NS2<R>{{sy[0] + an[0], sy[1] + an[1]}, {sy[0] - an[0], sy[1] - an[1]}});
+      NS2<R> e{{sy[0] + an[0], sy[1] + an[1]}, {sy[0] - an[0], sy[1] - an[1]}};
+      if (KIND == INV_SCALAR && a.dlam) {
+        // d/dlambda: (x + i y) i m = -m y + i m x
+        const R fm = (R)m;
+        e = NS2<R>{{-e.n.y * fm, e.n.x * fm}, {-e.s.y * fm, e.s.x * fm}};
+      }
+      put(pl, fl, m, e);
     }
   }
   __syncthreads();

@@ -55,6 +55,7 @@ struct InvArgs {
   int vor0, div0, u0, v0, nlev;         // INV_UV
   double plavor;
   int var;          // tile variant (0 default, 1 half the fields per block)
+  int dlam;         // INV_SCALAR: times i m before the FFT (d/dlambda)
   // MODE_BF16X3: the pre-split spectral operand (launch_pack) of fields
   // [bpk_f0, ...), bpk_stride bf16 values per member
   const void* bpk;
@@ -77,6 +78,25 @@ struct DirxTerm {
   int8_t in, ic, oc, sign;
   int8_t table;
 };
+// The term lists of the multi-input direct transforms (folded branches of
+// src/legmod.f90): {input, input component, output component, sign, table}
+// mktend (:538-549, fu <- gvz, fv <- guz as gridpointa calls it):
+//   D = qq.ke - qe.fv + i qm.fu     inputs (ke, fv, fu)
+//   T = qe.vt + qc.tn - i qm.ut     inputs (vt, tn, ut)
+//   Z = qe.fu + i qm.fv             inputs (fu, fv)
+// qtend (:468-496): q = qe.vq + qc.qn - i qm.uq   inputs (qn, uq, vq)
+inline constexpr DirxTerm MKTEND_D[6] = {{0, 0, 0, +1, TQQ}, {1, 0, 0, -1, TQE},
+                                         {2, 1, 0, -1, TQM}, {0, 1, 1, +1, TQQ},
+                                         {1, 1, 1, -1, TQE}, {2, 0, 1, +1, TQM}};
+inline constexpr DirxTerm MKTEND_T[6] = {{0, 0, 0, +1, TQE}, {1, 0, 0, +1, TQC},
+                                         {2, 1, 0, +1, TQM}, {0, 1, 1, +1, TQE},
+                                         {1, 1, 1, +1, TQC}, {2, 0, 1, -1, TQM}};
+inline constexpr DirxTerm MKTEND_Z[4] = {{0, 0, 0, +1, TQE}, {1, 1, 0, -1, TQM},
+                                         {0, 1, 1, +1, TQE}, {1, 0, 1, +1, TQM}};
+inline constexpr DirxTerm QTEND_Q[6] = {{2, 0, 0, +1, TQE}, {0, 0, 0, +1, TQC},
+                                        {1, 1, 0, +1, TQM}, {2, 1, 1, +1, TQE},
+                                        {0, 1, 1, +1, TQC}, {1, 0, 1, -1, TQM}};
+
 struct DirxArgs {
   const void* in[3];  // Fourier inputs fc(2, NLON/2, NLAT, nlev) per member
   void* out;          // spectral output (2*NCSP, nlev) per member
@@ -98,7 +118,9 @@ int launch_inverse_rt(const Tables& T, int mode, const InvArgs& aq, const InvArg
 int launch_forward(const Tables& T, int mode, bool fused, const FwdArgs& a, void* scratch,
                    size_t scratch_bytes, hipStream_t s);
 size_t forward_scratch_bytes(const Tables& T, int mode, int nmem, int nfield);
-int launch_fft(const Tables& T, int mode, int dir, void* a, int n, long lot, hipStream_t s);
+// in-place FFTs of lot rows of n values, for nmem members mstride values apart
+int launch_fft(const Tables& T, int mode, int dir, void* a, int n, long lot, hipStream_t s,
+               int nmem = 1, long mstride = 0);
 // pre-split copy (PK layout, sht_common.h) of fp32 spectral fields [f0, f0+nf)
 // of nmem members for the split-BF16 inverse; out holds nmem * nf *
 // pk_field(ntru) bf16 values

@@ -27,8 +27,12 @@ namespace exo {
 // FFT of independent rows (in place), two real rows per complex FFT
 // DIR +1: Fourier -> grid (fc2gp), DIR -1: grid -> Fourier (gp2fc)
 // ---------------------------------------------------------------------------
+// blockIdx.y selects a member: its rows start mstride values after the
+// previous member's
 template <int N, typename R, int DIR>
-__global__ __launch_bounds__(256) void k_fft_rows(R* a, long lot, const C2<R>* tw) {
+__global__ __launch_bounds__(256) void k_fft_rows(R* a, long lot, const C2<R>* tw,
+                                                  long mstride) {
+  a += blockIdx.y * mstride;
   constexpr int NF = FftBatch<N>::NF;
   __shared__ C2<R> fbuf[fft_buf_elems(NF * N)];
   __shared__ C2<R> tws[N];
@@ -135,14 +139,15 @@ size_t forward_scratch_bytes(const Tables& tb, int mode, int nmem, int nfield) {
 }
 
 template <int N, typename R>
-static int fft_t(const void* tw, int dir, R* a, long lot, hipStream_t s) {
+static int fft_t(const void* tw, int dir, R* a, long lot, int nmem, long mstride, hipStream_t s) {
   constexpr int NF = FftBatch<N>::NF;
   const long nblk = (lot + 2 * NF - 1) / (2 * NF);
-  if (nblk <= 0) return 0;
+  if (nblk <= 0 || nmem <= 0) return 0;
+  const dim3 grid((unsigned)nblk, (unsigned)nmem);
   if (dir > 0)
-    k_fft_rows<N, R, +1><<<(unsigned)nblk, 256, 0, s>>>(a, lot, (const C2<R>*)tw);
+    k_fft_rows<N, R, +1><<<grid, 256, 0, s>>>(a, lot, (const C2<R>*)tw, mstride);
   else
-    k_fft_rows<N, R, -1><<<(unsigned)nblk, 256, 0, s>>>(a, lot, (const C2<R>*)tw);
+    k_fft_rows<N, R, -1><<<grid, 256, 0, s>>>(a, lot, (const C2<R>*)tw, mstride);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
@@ -209,18 +214,19 @@ int launch_pack(const Tables& tb, const float* sp, long sp_stride, int f0, int n
   }
 }
 
-int launch_fft(const Tables& tb, int mode, int dir, void* a, int n, long lot, hipStream_t s) {
+int launch_fft(const Tables& tb, int mode, int dir, void* a, int n, long lot, hipStream_t s,
+               int nmem, long mstride) {
   if (n != tb.nlon) return -2;
   const bool d = (mode == MODE_F64);
   switch (n) {
-    case 64: return d ? fft_t<64, double>(tb.tw64, dir, (double*)a, lot, s)
-                      : fft_t<64, float>(tb.tw32, dir, (float*)a, lot, s);
-    case 128: return d ? fft_t<128, double>(tb.tw64, dir, (double*)a, lot, s)
-                       : fft_t<128, float>(tb.tw32, dir, (float*)a, lot, s);
-    case 192: return d ? fft_t<192, double>(tb.tw64, dir, (double*)a, lot, s)
-                       : fft_t<192, float>(tb.tw32, dir, (float*)a, lot, s);
-    case 384: return d ? fft_t<384, double>(tb.tw64, dir, (double*)a, lot, s)
-                       : fft_t<384, float>(tb.tw32, dir, (float*)a, lot, s);
+    case 64: return d ? fft_t<64, double>(tb.tw64, dir, (double*)a, lot, nmem, mstride, s)
+                      : fft_t<64, float>(tb.tw32, dir, (float*)a, lot, nmem, mstride, s);
+    case 128: return d ? fft_t<128, double>(tb.tw64, dir, (double*)a, lot, nmem, mstride, s)
+                       : fft_t<128, float>(tb.tw32, dir, (float*)a, lot, nmem, mstride, s);
+    case 192: return d ? fft_t<192, double>(tb.tw64, dir, (double*)a, lot, nmem, mstride, s)
+                       : fft_t<192, float>(tb.tw32, dir, (float*)a, lot, nmem, mstride, s);
+    case 384: return d ? fft_t<384, double>(tb.tw64, dir, (double*)a, lot, nmem, mstride, s)
+                       : fft_t<384, float>(tb.tw32, dir, (float*)a, lot, nmem, mstride, s);
     default: return -2;
   }
 }

@@ -7,8 +7,9 @@
   its heating-rate error.
 * ``LwEmulator``: a 43 -> 256 -> 256 -> 256 -> 21 GELU MLP whose forward and
   backward passes run on this package's own BF16 MFMA kernels
-  (exo_mlp_forward / exo_mlp_backward_dx / exo_mlp_backward_dw), wrapped as a
-  torch autograd op; FP32 master weights, Adam.  Inputs per column: 20-level
+  (exo_mlp_forward / exo_mlp_backward_dx / exo_mlp_backward_dw), registered
+  as torch operators (exoplasim_amd::lw_mlp_forward / lw_mlp_backward) with
+  autograd; FP32 master weights, Adam.  Inputs per column: 20-level
   temperature, 20-level specific humidity, surface temperature, surface
   pressure, CO2; outputs: 20-level heating rates (K/day) and OLR (W/m2).
 
@@ -73,6 +74,9 @@ def _proto():
         L.exo_mlp_cast_rows.argtypes = [fp, c_int, c_int, c_int, vp, vp]
         L.exo_lw_emulate.restype = c_int
         L.exo_lw_emulate.argtypes = [POINTER(_Net), c_int, fp, fp, fp, fp, fp, fp, vp]
+        L.exo_lw_emulate_guarded.restype = c_int
+        L.exo_lw_emulate_guarded.argtypes = [POINTER(_Net), fp, fp, c_int, fp, fp, fp, fp, fp,
+                                             fp, vp, vp]
         L._rad_proto = True
     return L
 
@@ -197,86 +201,132 @@ def _ptr(t):
     return t.data_ptr()
 
 
-class _EmulatorOp:
-    """The network forward/backward on the HIP kernels as a torch autograd
-    function of (features, W1, b1, ..., W4, b4)."""
+def _register_ops():
+    """The network's forward and backward passes as torch operators
+    (torch.library custom ops ``exoplasim_amd::lw_mlp_forward`` and
+    ``exoplasim_amd::lw_mlp_backward``) on this package's HIP kernels, with
+    autograd and shape functions registered: the returned function
+    ``(x, W1, b1, ..., W4, b4) -> y`` is differentiable in the parameters."""
+    import torch
 
-    @staticmethod
-    def make():
-        import torch
+    def _weights(params, dev):
+        L, s = _proto(), _stream()
+        wb, wtb = [], []
+        for li, ((nin, nout), (K, N)) in enumerate(zip(LAYERS, PADDED)):
+            w = params[li].contiguous()
+            W = torch.empty((N, K), dtype=torch.bfloat16, device=dev)
+            Wt = torch.empty((K, N), dtype=torch.bfloat16, device=dev)
+            _chk(L.exo_mlp_cast_weights(_ptr(w), nout, nin, N, K, _ptr(W), _ptr(Wt), s),
+                 "cast_weights")
+            wb.append(W)
+            wtb.append(Wt)
+        return wb, wtb
 
-        class Op(torch.autograd.Function):
-            @staticmethod
-            def forward(ctx, x, *params):
-                L = _proto()
-                s = _stream()
-                M = x.shape[0]
-                buf = _MLP(M, x.device)
-                wb, wtb = [], []
-                for li, ((nin, nout), (K, N)) in enumerate(zip(LAYERS, PADDED)):
-                    w = params[2 * li].contiguous()
-                    W = torch.empty((N, K), dtype=torch.bfloat16, device=x.device)
-                    Wt = torch.empty((K, N), dtype=torch.bfloat16, device=x.device)
-                    _chk(L.exo_mlp_cast_weights(_ptr(w), nout, nin, N, K, _ptr(W), _ptr(Wt), s),
-                         "cast_weights")
-                    wb.append(W)
-                    wtb.append(Wt)
-                xf = x.float().contiguous()
-                _chk(L.exo_mlp_cast_rows(_ptr(xf), M, NIN, KIN, _ptr(buf.x0), s), "cast_rows")
-                inp = buf.x0
-                for li in range(3):
-                    K, N = PADDED[li]
-                    _chk(L.exo_mlp_forward(_ptr(inp), _ptr(wb[li]), _ptr(params[2 * li + 1]), M, K,
-                                           N, LAYERS[li][1], 1, _ptr(buf.z[li]), _ptr(buf.h[li]),
-                                           None, s), "mlp_forward")
-                    inp = buf.h[li]
-                _chk(L.exo_mlp_forward(_ptr(inp), _ptr(wb[3]), _ptr(params[7]), M, NHID, NOUTP,
-                                       NOUT, 0, None, None, _ptr(buf.y), s), "mlp_forward")
-                ctx.buf, ctx.wtb = buf, wtb
-                return buf.y[:, :NOUT].clone()
+    t8 = "(" + ", ".join(["Tensor"] * 8) + ")"
 
-            @staticmethod
-            def backward(ctx, gy):
-                L = _proto()
-                s = _stream()
-                buf, wtb = ctx.buf, ctx.wtb
-                M = buf.M
-                dev = gy.device
-                gy32 = torch.zeros((M, NOUTP), dtype=torch.float32, device=dev)
-                gy32[:, :NOUT] = gy
-                _chk(L.exo_mlp_cast_rows(_ptr(gy32), M, NOUTP, NOUTP, _ptr(buf.dy), s), "cast_rows")
-                grads = [None] * 8
-                dz, dz32 = buf.dy, gy32
-                for li in (3, 2, 1, 0):
-                    K, N = PADDED[li]
-                    nin, nout = LAYERS[li]
-                    xin = buf.h[li - 1] if li > 0 else buf.x0
-                    dW = torch.empty((N, K), dtype=torch.float32, device=dev)
-                    db = torch.empty(N, dtype=torch.float32, device=dev)
-                    _chk(L.exo_mlp_backward_dw(_ptr(dz), _ptr(xin), _ptr(dz32), M, N, K,
-                                               buf.nsplit, _ptr(buf.part), _ptr(dW), _ptr(db), s),
-                         "mlp_backward_dw")
-                    grads[2 * li] = dW[:nout, :nin].contiguous()
-                    grads[2 * li + 1] = db[:nout].contiguous()
-                    if li > 0:
-                        Kp = PADDED[li - 1][1]
-                        _chk(L.exo_mlp_backward_dx(_ptr(dz), _ptr(wtb[li]), _ptr(buf.z[li - 1]), M,
-                                                   N, Kp, LAYERS[li - 1][1], _ptr(buf.dz[li - 1]),
-                                                   _ptr(buf.dz32[li - 1]), s),
-                             "mlp_backward_dx")
-                        dz, dz32 = buf.dz[li - 1], buf.dz32[li - 1]
-                return (None, *grads)
+    @torch.library.custom_op("exoplasim_amd::lw_mlp_forward", mutates_args=(),
+                             schema="(Tensor x, Tensor w1, Tensor b1, Tensor w2, Tensor b2, "
+                                    "Tensor w3, Tensor b3, Tensor w4, Tensor b4) -> " + t8)
+    def lw_mlp_forward(x, w1, b1, w2, b2, w3, b3, w4, b4):
+        """y (M, 21) and the activations the backward pass reads: the bf16
+        inputs x0, the pre-activations z1..z3 (fp32) and the bf16 hidden
+        layers h1..h3"""
+        L, s = _proto(), _stream()
+        bias = (b1, b2, b3, b4)
+        M = x.shape[0]
+        buf = _MLP(M, x.device)
+        wb, _ = _weights((w1, w2, w3, w4), x.device)
+        xf = x.float().contiguous()
+        _chk(L.exo_mlp_cast_rows(_ptr(xf), M, NIN, KIN, _ptr(buf.x0), s), "cast_rows")
+        inp = buf.x0
+        for li in range(3):
+            K, N = PADDED[li]
+            _chk(L.exo_mlp_forward(_ptr(inp), _ptr(wb[li]), _ptr(bias[li]), M, K, N,
+                                   LAYERS[li][1], 1, _ptr(buf.z[li]), _ptr(buf.h[li]), None,
+                                   s), "mlp_forward")
+            inp = buf.h[li]
+        _chk(L.exo_mlp_forward(_ptr(inp), _ptr(wb[3]), _ptr(b4), M, NHID, NOUTP, NOUT, 0, None,
+                               None, _ptr(buf.y), s), "mlp_forward")
+        return (buf.y[:, :NOUT].clone(), buf.x0, *buf.z, *buf.h)
 
-        return Op
+    @lw_mlp_forward.register_fake
+    def _(x, w1, b1, w2, b2, w3, b3, w4, b4):
+        M = x.shape[0]
+        f32, bf = torch.float32, torch.bfloat16
+        return (x.new_empty((M, NOUT), dtype=f32), x.new_empty((M, KIN), dtype=bf),
+                *[x.new_empty((M, NHID), dtype=f32) for _ in range(3)],
+                *[x.new_empty((M, NHID), dtype=bf) for _ in range(3)])
+
+    @torch.library.custom_op("exoplasim_amd::lw_mlp_backward", mutates_args=(),
+                             schema="(Tensor gy, Tensor x0, Tensor z1, Tensor z2, Tensor z3, "
+                                    "Tensor h1, Tensor h2, Tensor h3, Tensor w1, Tensor w2, "
+                                    "Tensor w3, Tensor w4) -> " + t8)
+    def lw_mlp_backward(gy, x0, z1, z2, z3, h1, h2, h3, w1, w2, w3, w4):
+        """gradients (dW1, db1, ..., dW4, db4) for the output gradient gy"""
+        L, s = _proto(), _stream()
+        M = gy.shape[0]
+        dev = gy.device
+        buf = _MLP(M, dev)
+        z, h = (z1, z2, z3), (h1, h2, h3)
+        _, wtb = _weights((w1, w2, w3, w4), dev)
+        gy32 = torch.zeros((M, NOUTP), dtype=torch.float32, device=dev)
+        gy32[:, :NOUT] = gy
+        _chk(L.exo_mlp_cast_rows(_ptr(gy32), M, NOUTP, NOUTP, _ptr(buf.dy), s), "cast_rows")
+        grads = [None] * 8
+        dz, dz32 = buf.dy, gy32
+        for li in (3, 2, 1, 0):
+            K, N = PADDED[li]
+            nin, nout = LAYERS[li]
+            xin = h[li - 1] if li > 0 else x0
+            dW = torch.empty((N, K), dtype=torch.float32, device=dev)
+            db = torch.empty(N, dtype=torch.float32, device=dev)
+            _chk(L.exo_mlp_backward_dw(_ptr(dz), _ptr(xin), _ptr(dz32), M, N, K, buf.nsplit,
+                                       _ptr(buf.part), _ptr(dW), _ptr(db), s),
+                 "mlp_backward_dw")
+            grads[2 * li] = dW[:nout, :nin].contiguous()
+            grads[2 * li + 1] = db[:nout].contiguous()
+            if li > 0:
+                Kp = PADDED[li - 1][1]
+                _chk(L.exo_mlp_backward_dx(_ptr(dz), _ptr(wtb[li]), _ptr(z[li - 1]), M, N, Kp,
+                                           LAYERS[li - 1][1], _ptr(buf.dz[li - 1]),
+                                           _ptr(buf.dz32[li - 1]), s), "mlp_backward_dx")
+                dz, dz32 = buf.dz[li - 1], buf.dz32[li - 1]
+        return tuple(grads)
+
+    @lw_mlp_backward.register_fake
+    def _(gy, x0, z1, z2, z3, h1, h2, h3, w1, w2, w3, w4):
+        out = []
+        for w in (w1, w2, w3, w4):
+            out += [torch.empty_like(w), w.new_empty((w.shape[0],))]
+        return tuple(out)
+
+    def setup_context(ctx, inputs, output):
+        ctx.save_for_backward(*output[1:], inputs[1], inputs[3], inputs[5], inputs[7])
+
+    def backward(ctx, gy, *unused):
+        x0, z1, z2, z3, h1, h2, h3, w1, w2, w3, w4 = ctx.saved_tensors
+        g = torch.ops.exoplasim_amd.lw_mlp_backward(gy.contiguous(), x0, z1, z2, z3, h1, h2, h3,
+                                                    w1, w2, w3, w4)
+        return (None, *g)
+
+    torch.library.register_autograd("exoplasim_amd::lw_mlp_forward", backward,
+                                    setup_context=setup_context)
+
+    def apply(x, *params):
+        return torch.ops.exoplasim_amd.lw_mlp_forward(x, *params)[0]
+
+    return apply
 
 
 _OP = None
 
 
 def emulator_op():
+    """the network as a differentiable function of (features, W1, b1, ...,
+    W4, b4), through the registered torch operators"""
     global _OP
     if _OP is None:
-        _OP = _EmulatorOp.make()
+        _OP = _register_ops()
     return _OP
 
 
@@ -296,6 +346,9 @@ class LwEmulator:
                                                                    requires_grad=True)]
         self.ymean = torch.zeros(NOUT, device="cuda")
         self.ystd = torch.ones(NOUT, device="cuda")
+        # the training envelope in feature space (set by fit)
+        self.flo = torch.full((NIN,), -float("inf"), device="cuda")
+        self.fhi = torch.full((NIN,), float("inf"), device="cuda")
         self._net = None  # bf16 inference weights, rebuilt after training
 
     def targets(self, heat, olr, cols):
@@ -307,7 +360,7 @@ class LwEmulator:
         return torch.cat([(heat.float() * ps[None, :]).t(), olr.float()[:, None]], dim=1)
 
     def raw(self, x):
-        return emulator_op().apply(x, *self.params)
+        return emulator_op()(x, *self.params)
 
     def _freeze(self):
         """bf16 padded copies of the weights for the fused inference kernel"""
@@ -353,6 +406,40 @@ class LwEmulator:
              "exo_lw_emulate")
         return heat, olr
 
+    def predict_guarded(self, cols, column: "LwrColumn"):
+        """heating and OLR like predict, with every column whose inputs lie
+        outside the training envelope computed by the plain long-wave column
+        instead (the reference's own lwr, src/radmod.f90:2122, and heating
+        rate, :1107-1108).  Returns (heat, olr, number of plain columns)."""
+        import torch
+
+        if self._net is None:
+            self._freeze()
+
+        def f32(t):
+            return t if (t.dtype == torch.float32 and t.is_contiguous()) else t.float().contiguous()
+
+        dt, dq, dp, co2 = f32(cols["dt"]), f32(cols["dq"]), f32(cols["dp"]), f32(cols["dqco2"])
+        ncol = dp.shape[0]
+        heat = torch.empty((self.nlev, ncol), dtype=torch.float32, device=dp.device)
+        olr = torch.empty(ncol, dtype=torch.float32, device=dp.device)
+        outside = torch.empty(ncol, dtype=torch.int32, device=dp.device)
+        flo, fhi = self.flo.float().contiguous(), self.fhi.float().contiguous()
+        _chk(_proto().exo_lw_emulate_guarded(self._net_ref, flo.data_ptr(), fhi.data_ptr(), ncol,
+                                             dt.data_ptr(), dq.data_ptr(), dp.data_ptr(),
+                                             co2.data_ptr(), heat.data_ptr(), olr.data_ptr(),
+                                             outside.data_ptr(), _stream()),
+             "exo_lw_emulate_guarded")
+        idx = torch.nonzero(outside, as_tuple=True)[0]
+        n = int(idx.numel())
+        if n:
+            sub = {k: (v[..., idx] if v.dim() > 1 else v[idx]).contiguous()
+                   for k, v in cols.items()}
+            h, o = column(sub)
+            heat[:, idx] = h.float()
+            olr[idx] = o.float()
+        return heat, olr, n
+
     def predict_layers(self, cols):
         """the same through the per-layer training kernels (features in torch)"""
         import torch
@@ -368,6 +455,10 @@ class LwEmulator:
         import torch
 
         self._net = None
+        # the envelope of the training inputs, widened by 2% of each range
+        lo, hi = x.min(dim=0).values.float(), x.max(dim=0).values.float()
+        pad = 0.02 * (hi - lo)
+        self.flo, self.fhi = (lo - pad).contiguous(), (hi + pad).contiguous()
         self.ymean = y.mean(dim=0)
         self.ystd = y.std(dim=0).clamp_min(1e-6)
         yn = (y - self.ymean) / self.ystd
@@ -392,5 +483,5 @@ class LwEmulator:
                 tot += float(loss.detach()) if b % 50 == 0 else 0.0
                 it += 1
             if log:
-                log(f"epoch {ep}: loss {float(loss):.4e}")
-        return float(loss)
+                log(f"epoch {ep}: loss {float(loss.detach()):.4e}")
+        return float(loss.detach())

@@ -87,6 +87,14 @@ typedef struct {
  * per block of 32 columns with the activations in LDS. */
 int exo_lw_emulate(const exo_lw_net *net, int ncol, const float *dt, const float *dq,
                    const float *dp, const float *dqco2, float *heat, float *olr, void *stream);
+/* The same, and outside[col] = 1 for every column with one of its 43
+ * network inputs outside [flo, fhi] (the training envelope) or not a
+ * number, else 0: the columns the caller hands to the plain column exo_lwr
+ * instead, as the reference would have computed all of them. */
+int exo_lw_emulate_guarded(const exo_lw_net *net, const float *flo, const float *fhi, int ncol,
+                           const float *dt, const float *dq, const float *dp,
+                           const float *dqco2, float *heat, float *olr, int *outside,
+                           void *stream);
 
 #ifdef __cplusplus
 }

@@ -152,6 +152,18 @@ class Oracle:
                             _p(c(vt)), _p(d), _p(t), _p(z))
         return d, t, z
 
+    def fc2sp_unfolded(self, ntru, nlon, qc, sk, fc):
+        """fc2sp's multi-process branch (src/legmod.f90:191-202) over the
+        latitudes of qc (nlpp, ncsp) and fc (nlpp, nlon): partial sums"""
+        ncsp = (ntru + 1) * (ntru + 2) // 2
+        qc = np.ascontiguousarray(qc, dtype=np.float64)
+        fc = np.ascontiguousarray(fc, dtype=np.float64)
+        sp = np.zeros(2 * ncsp)
+        self.lib.ora_fc2sp_unfolded(c_int(ntru), c_int(qc.shape[0]), c_int(nlon), _p(qc),
+                                    _p(np.ascontiguousarray(sk, dtype=np.float64)), _p(fc),
+                                    _p(sp))
+        return sp
+
     def gp2fc(self, a: np.ndarray, dft: bool = False) -> np.ndarray:
         a = np.array(a, dtype=np.float64, order="C")
         n = a.shape[-1]
@@ -273,6 +285,121 @@ class Oracle:
                             c_int(nqspec), *[_p(out[k]) for k in
                                              ("gtn", "gqn", "guz", "gvz", "gvp", "gphi", "dw")])
         return out
+
+    def gridpointa(self, ntru, nlev, state, v, scal, nqspec=1, gq=None, dq=None, sk=None,
+                   plavor=PLAVOR):
+        """The transform group of gridpointa (src/plasim.f90:2402-2486, its
+        diagnostics left out) for one member, composed of the oracle's
+        restatements in the reference's order: invlega (src/legmod.f90:
+        635-656), gpmt (:2425-2430), fc2gp, gp = exp(gp), calcgp, the products
+        (:2436-2446), gp2fc, fc2sp(gvpp), mktend (fu <- gvz, fv <- guz) and
+        qtend.  state: (4*nlev+1, nrsp) = sz sd st sq sp; with nqspec = 0 the
+        grid gq and dq (nlev, nhor) are the caller's.  Returns the
+        tendencies in the same layout (sqt zero when nqspec = 0) and the grid
+        fields (dict)."""
+        nlat, nlon, ncsp = dims(ntru)
+        n, nh = nlev, nlat * nlon
+        sk = np.ones(ntru + 1) if sk is None else sk
+        t = self.legini(ntru, nlat)
+        sz, sd, st, sq, sp = (state[:n], state[n:2 * n], state[2 * n:3 * n],
+                              state[3 * n:4 * n], state[4 * n])
+        fc = lambda x, q="qi": self.sp2fc(ntru, nlat, nlon, t[q], sk, x)  # noqa: E731
+        lev = lambda a: np.stack([fc(a[j]) for j in range(n)])  # noqa: E731
+        dft = nlon == 192  # T63 runs fft991 in the reference build (pkg/compile.sh:103-108)
+        grid = lambda f: self.fc2gp(f, dft).reshape(f.shape[0] if f.ndim == 3 else 1, nh)  # noqa
+        pu, pv = self.dv2uv(ntru, nlat, nlon, n, t["qu"], t["qv"], sk, plavor, sd, sz)
+        f = {"gd": grid(lev(sd)), "gt": grid(lev(st)), "gz": grid(lev(sz)),
+             "gu": grid(pu), "gv": grid(pv)}
+        f["gq"] = grid(lev(sq)) if nqspec == 1 else np.asarray(gq, dtype=np.float64)
+        f["dq"] = np.zeros((n, nh)) if dq is None else np.asarray(dq, dtype=np.float64)
+        fcp = fc(sp)
+        m = np.arange(nlon // 2)
+        fm = np.empty_like(fcp)
+        fm[:, 0::2] = -fcp[:, 1::2] * m
+        fm[:, 1::2] = fcp[:, 0::2] * m
+        f["gpm"] = grid(fm)[0]
+        f["gpj"] = grid(fc(sp, "qj"))[0]
+        f["gp"] = np.exp(grid(fcp)[0])
+        sid, _ = self.inigau(nlat)
+        f["rcsq"] = np.repeat(1.0 / (1.0 - sid * sid), nlon)
+        out = self.calcgp(f, v, scal, nqspec)
+        f.update(out)
+        f["gut"], f["gvt"] = f["gu"] * f["gt"], f["gv"] * f["gt"]
+        f["gke"] = f["gu"] * f["gu"] + f["gv"] * f["gv"] + out["gphi"]
+        if nqspec == 1:
+            f["guq"], f["gvq"] = f["gu"] * f["gq"], f["gv"] * f["gq"]
+        four = lambda a: self.gp2fc(a.reshape(-1, nlat, nlon), dft)  # noqa: E731
+        F = {k: four(f[k]) for k in ("gtn", "gqn", "gut", "gvt", "guz", "gvz", "gke")}
+        tend = np.zeros((4 * n + 1, 2 * ncsp))
+        tend[4 * n] = self.fc2sp(ntru, nlat, nlon, t["qc"], sk, four(out["gvp"])[0])
+        d, tt, z = self.mktend(ntru, nlat, nlon, n, t["qc"], t["qe"], t["qm"], t["qq"], sk,
+                               F["gtn"], F["gvz"], F["guz"], F["gke"], F["gut"], F["gvt"])
+        tend[:n], tend[n:2 * n], tend[2 * n:3 * n] = z, d, tt
+        if nqspec == 1:
+            tend[3 * n:4 * n] = self.qtend(ntru, nlat, nlon, n, t["qc"], t["qe"], t["qm"], sk,
+                                           F["gqn"], four(f["guq"]), four(f["gvq"]))
+        return tend, f
+
+
+# ---------------------------------------------------------------------------
+# the multi-process pieces around the transforms
+# ---------------------------------------------------------------------------
+def writesp(pf: np.ndarray, ntru: int, pscale: float, poff: float, real=np.float64):
+    """writesp's values (src/outmod.f90:141-169) of one spectral field pf
+    (NRSP): zf = real(4)(pf * spnorm * pscale), zf(1) = zf(1) + poff, with
+    spnorm = +-1/sqrt(2) alternating with m (src/plasim.f90:1638-1651), all
+    in the model's real kind `real`"""
+    norm = real(1.0) / np.sqrt(real(2.0))
+    spn = []
+    for m in range(ntru + 1):
+        spn += [norm] * (2 * (ntru + 1 - m))
+        norm = -norm
+    spn = np.array(spn, dtype=real)
+    zf = ((np.asarray(pf, dtype=real) * spn) * real(pscale)).astype(np.float32)
+    zf[0] = np.float32(real(zf[0]) + real(poff))
+    return zf
+
+
+def sumsc(psf: np.ndarray, nspp: int) -> np.ndarray:
+    """mpsumsc (src/mpimod.f90:279-291): MPI_Reduce_scatter with counts
+    nscatsp = NSPP (src/plasimmod.f90:306) over the processes' partial sums
+    psf (nproc, nfield, NESP); process r receives elements r NSPP .. of the
+    sum, here summed in process order."""
+    nproc = psf.shape[0]
+    out = np.zeros((nproc, psf.shape[1], nspp))
+    for r in range(nproc):
+        acc = psf[0, :, r * nspp:(r + 1) * nspp].copy()
+        for q in range(1, nproc):
+            acc = acc + psf[q, :, r * nspp:(r + 1) * nspp]
+        out[r] = acc
+    return out
+
+
+def mrtrunc(sphi: np.ndarray, nthi: int, ntlo: int) -> np.ndarray:
+    """mrtrunc (src/mpimod_multi.f90:117-131) of one field sphi(2, *)"""
+    splo = np.zeros(2 * (ntlo + 1) * (ntlo + 2) // 2)
+    jl = jh = 0
+    for jm in range(ntlo + 1):
+        jn = ntlo - jm
+        splo[2 * jl:2 * (jl + jn + 1)] = sphi[2 * jh:2 * (jh + jn + 1)]
+        jl += ntlo - jm + 1
+        jh += nthi - jm + 1
+    return splo
+
+
+def mrexpand(splo: np.ndarray, ntlo: int, sphi: np.ndarray, nthi: int) -> np.ndarray:
+    """mrexpand (src/mpimod_multi.f90:137-153) into a copy of sphi: the
+    reference clears sphi(:, 1:nthi), the first nthi complex values, then
+    copies the low truncation in"""
+    out = np.array(sphi, dtype=np.float64, copy=True)
+    out[:2 * nthi] = 0.0
+    jl = jh = 0
+    for jm in range(ntlo + 1):
+        jn = ntlo - jm
+        out[2 * jh:2 * (jh + jn + 1)] = splo[2 * jl:2 * (jl + jn + 1)]
+        jl += ntlo - jm + 1
+        jh += nthi - jm + 1
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -484,6 +611,10 @@ class PyfftOracle:
     (pyfft.f90:527-545), one filter serves both directions, and fc2gp ends
     with a multiplication by sqrt 2 (pyfft.f90:891).
 
+    Rows of 192 (T63) run the FFT991 definition (the post-processor's
+    pyfft991.f90 copy; pyfft.f90's own FFT stops for that length,
+    pyfft.f90:791-792).
+
     kind "r4" evaluates the default-real pieces as the post-processor's
     default-real build does: the sqrt 2 literal and the filter formula
     (pyfft.f90:150-153) in single precision.  kind "r8" evaluates them in
@@ -518,6 +649,12 @@ class PyfftOracle:
         expf = _libm().expf
         return np.array([expf(float(np.float32(-8.0) * v)) for v in p], dtype=np.float64)
 
+    def _fc2gp(self, a):
+        return self.o.fc2gp(a, dft=np.shape(a)[-1] == 192)
+
+    def _gp2fc(self, a):
+        return self.o.gp2fc(a, dft=np.shape(a)[-1] == 192)
+
     def _t(self, ntru, nlat):
         if (ntru, nlat) not in self._tabs:
             self._tabs[(ntru, nlat)] = self.o.legini(ntru, nlat)
@@ -527,13 +664,13 @@ class PyfftOracle:
         sp = np.atleast_2d(sp)
         t, f = self._t(ntru, nlat), self.sfilt(ntru, nfs)
         fc = np.stack([self.o.sp2fc(ntru, nlat, 2 * nlat, t["qi"], f, x) for x in sp])
-        return self.o.fc2gp(fc) * self.sqrt2
+        return self._fc2gp(fc) * self.sqrt2
 
     def gp2sp(self, ntru, gp, nfs=0):
         gp = np.asarray(gp, dtype=np.float64)
         nlat = gp.shape[1]
         t, f = self._t(ntru, nlat), self.sfilt(ntru, nfs)
-        fc = self.o.gp2fc(gp / self.sqrt2)
+        fc = self._gp2fc(gp / self.sqrt2)
         return np.stack([self.o.fc2sp(ntru, nlat, 2 * nlat, t["qc"], f, x) for x in fc])
 
     def sp3fc(self, ntru, nlat, sp, nfs=0):
@@ -542,10 +679,10 @@ class PyfftOracle:
         return np.stack([self.o.sp2fc(ntru, nlat, 2 * nlat, t["qi"], f, x) for x in sp])
 
     def gp3fc(self, gp):
-        return self.o.gp2fc(gp)
+        return self._gp2fc(gp)
 
     def fc3gp(self, fc):
-        return self.o.fc2gp(fc) * self.sqrt2
+        return self._fc2gp(fc) * self.sqrt2
 
     def spvgp(self, ntru, sd, sz, rdcostheta, nfs=0):
         sd, sz = np.atleast_2d(sd), np.atleast_2d(sz)
@@ -554,16 +691,16 @@ class PyfftOracle:
         t, f = self._t(ntru, nlat), self.sfilt(ntru, nfs)
         pu, pv = self.o.dv2uv(ntru, nlat, 2 * nlat, sd.shape[0], t["qu"], t["qv"], f, 0.0,
                               sd, sz)
-        gu = (self.o.fc2gp(pu) * self.sqrt2) * rd[None, :, None]
-        gv = (self.o.fc2gp(pv) * self.sqrt2) * rd[None, :, None]
+        gu = (self._fc2gp(pu) * self.sqrt2) * rd[None, :, None]
+        gv = (self._fc2gp(pv) * self.sqrt2) * rd[None, :, None]
         return gu, gv
 
     def gpvsp(self, ntru, gu, gv, costhetadr, nfs=0):
         cd = np.asarray(costhetadr, dtype=np.float64)
         nlat = cd.size
         t, f = self._t(ntru, nlat), self.sfilt(ntru, nfs)
-        fu = self.o.gp2fc((np.asarray(gu) * cd[None, :, None]) / self.sqrt2)
-        fv = self.o.gp2fc((np.asarray(gv) * cd[None, :, None]) / self.sqrt2)
+        fu = self._gp2fc((np.asarray(gu) * cd[None, :, None]) / self.sqrt2)
+        fv = self._gp2fc((np.asarray(gv) * cd[None, :, None]) / self.sqrt2)
         return self.o.uv2dv(ntru, nlat, 2 * nlat, fu.shape[0], t["qe"], t["qm"], f, fu, fv)
 
 
@@ -638,6 +775,35 @@ class RefPlasim:
                                    *[_p(out[k]) for k in ("gtn", "gqn", "guz", "gvz", "gvp",
                                                           "gphi", "dw")])
         return out
+
+
+class RefGridpointa:
+    """The reference's gridpointa (src/plasim.f90:2360-2585) with legmod,
+    gaussmod and fftmod at the default resolution module (T21 L10), FP64,
+    through oracle/ref_glue_gridpointa.f90; lazy binding as RefPlasim."""
+
+    def __init__(self):
+        self.lib = ctypes.CDLL(_ref_path("libref_gpa.so"), mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+        self.ntru, self.nlat, self.nlon, self.nlev = 21, 32, 64, 10
+
+    def run(self, state, v, scal, nqspec=1, gq=None, dq=None):
+        """state (4*nlev+1, nrsp) = sz sd st sq sp; returns the tendencies in
+        the same layout and the grid fields gz gd gt gu gv dw (nlev, nhor),
+        gp gpj (nhor,)"""
+        n, nh = self.nlev, self.nlat * self.nlon
+        c = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+        tend = np.zeros_like(c(state))
+        grid = np.zeros((6 * n + 2, nh))
+        gq = np.zeros((n, nh)) if gq is None else gq
+        dq = np.zeros((n, nh)) if dq is None else dq
+        self.lib.ref_gridpointa(_p(c(state)), *[_p(c(v[k])) for k in ("sigmah", "dsigma",
+                                                                       "rdsig", "t0")],
+                                byref(c_double(v["akap"])), _p(c(scal)), c_int(nqspec),
+                                _p(c(gq)), _p(c(dq)), _p(tend), _p(grid))
+        names = ("gz", "gd", "gt", "gu", "gv", "dw")
+        g = {k: grid[i * n:(i + 1) * n] for i, k in enumerate(names)}
+        g["gp"], g["gpj"] = grid[6 * n], grid[6 * n + 1]
+        return tend, g
 
 
 class RefFft991:

@@ -84,6 +84,46 @@ def main():
     plasim_golden()
     lwr_golden()
     pyfft_golden()
+    gridpointa_golden()
+
+
+GPA_ROWS = [0, 4, 9]  # levels of each tendency field kept in the golden file
+
+
+def gridpointa_state(ntru, nlev, salt=0.9):
+    """the spectral state sz sd st sq sp of the gridpointa pins: small
+    deterministic fields on the planetary vorticity, as a model state"""
+    nfl = 4 * nlev + 1
+    x = 0.01 * spectral_input(ntru, nfl, salt)
+    x[:nlev, 2] += 1.63299310207  # plavor at Re(m=0, n=1) of the absolute vorticity
+    x[3 * nlev:4 * nlev] *= 0.1   # humidity
+    return x
+
+
+def gridpointa_humidity(nlev, nhor):
+    """grid-point gq and dq of the nqspec = 0 pin"""
+    return (1e-3 * np.abs(golden_input((nlev, nhor), 5.3)),
+            1e-3 * golden_input((nlev, nhor), 5.9))
+
+
+def gridpointa_golden():
+    """tendencies of the reference's gridpointa (T21 L10) for nqspec 1 and 0"""
+    from oracle import Oracle, RefGridpointa
+
+    r = RefGridpointa()
+    n = r.nlev
+    v = plasim_vertical(Oracle(), n)
+    st = gridpointa_state(21, n)
+    gq, dq = gridpointa_humidity(n, r.nlat * r.nlon)
+    out = {}
+    for nq in (1, 0):
+        tend, grid = r.run(st, v, PLASIM_SCAL, nq, gq=gq, dq=dq)
+        rows = [f * n + j for f in range(4) for j in GPA_ROWS] + [4 * n]
+        out[f"tend{nq}"] = tend[rows]
+        out[f"gp{nq}"] = grid["gp"]
+    path = os.path.join(HERE, "reference_gridpointa_t21.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes")
 
 
 def pyfft_inputs(ntru, nlat, nlev):
@@ -276,5 +316,7 @@ if __name__ == "__main__":
         plasim_golden()
     elif sys.argv[1:] == ["pyfft"]:
         pyfft_golden()
+    elif sys.argv[1:] == ["gridpointa"]:
+        gridpointa_golden()
     else:
         main()

@@ -28,6 +28,18 @@ def test_header_symbols_exported():
     assert set(syms) == set(_lib.PROTOTYPES), set(syms) ^ set(_lib.PROTOTYPES)
 
 
+def test_other_headers_exported():
+    """every entry point of the radiation and dynamics headers is exported"""
+    from exoplasim_amd import _lib
+
+    lib = _lib.lib()
+    for h in ("exo_rad.h", "exo_dyn.h", "exo_mp.h", "exo_out.h"):
+        syms = declared_symbols(os.path.join(ROOT, "include", h))
+        assert len(syms) >= 1, h
+        missing = [s for s in syms if not hasattr(lib, s)]
+        assert not missing, (h, missing)
+
+
 def test_strerror_and_argument_errors():
     from exoplasim_amd import _lib
 

@@ -97,3 +97,39 @@ def test_pyfft_rejects_unsupported(gpu):
         pyfft.sp2gp(np.zeros((2 * 86 * 87 // 2, 1), order="F"), 128, 256, 85, 0)
     with pytest.raises(ExoError):
         pyfft.sp2gp(np.zeros((10, 1)), 32, 64, 21, 0)
+
+
+@pytest.mark.parametrize("mode", ["f64", "f32"])
+def test_writesp_and_the_postprocessor_chain(oracle, gpu, mode):
+    """writesp's output values (exoplasim_amd.output.writesp, the model's
+    spectral output format, src/outmod.f90:141-169) against the oracle's
+    restatement, bit for bit; then the post-processor's chain on them:
+    pyfft.sp2gp of the written coefficients, rolled by half the longitudes
+    as pyburn does (exoplasim/pyburn.py:711-713), is the model's own grid of
+    the field (sp2fc + fc2gp), up to the single-precision sqrt 2 literal."""
+    import torch
+
+    from oracle import writesp as ora_writesp
+
+    from exoplasim_amd import pyfft
+    from exoplasim_amd.output import writesp
+
+    ntru, nlat, nlev = 42, 64, 3
+    sp = pyfft_inputs(ntru, nlat, nlev)[0]
+    real = np.float64 if mode == "f64" else np.float32
+    dt = torch.float64 if mode == "f64" else torch.float32
+    poff = np.array([0.0, 2.5, -1.0])
+    z = writesp(mode, ntru, torch.from_numpy(sp.astype(real)).cuda(), 7.0,
+                torch.from_numpy(poff).cuda()).cpu().numpy()
+    for k in range(nlev):
+        np.testing.assert_array_equal(z[k], ora_writesp(sp[k].astype(real), ntru, 7.0, poff[k],
+                                                        real))
+    # the chain, on the scale-1, offset-free output
+    z1 = writesp(mode, ntru, torch.from_numpy(sp.astype(real)).cuda(), 1.0).cpu().numpy()
+    g = pyfft.sp2gp(F(z1.astype(np.float64)), nlat, 2 * nlat, ntru, 0)
+    g = np.roll(np.transpose(g, (2, 1, 0)), nlat, axis=-1)
+    t = oracle.legini(ntru, nlat)
+    one = np.ones(ntru + 1)
+    want = oracle.fc2gp(np.stack([oracle.sp2fc(ntru, nlat, 2 * nlat, t["qi"], one, x)
+                                  for x in sp]))
+    assert rel(g, want) < (1e-7 if mode == "f64" else 3e-7)

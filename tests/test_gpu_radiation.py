@@ -144,7 +144,7 @@ def test_emulator_training_and_gain(gpu):
     from exoplasim_amd.radiation import LwEmulator, LwrColumn, columns_to_device, features
 
     nlev = 20
-    ntrain = 400_000
+    ntrain = 2_000_000
     print("emulator training", flush=True)
     col = LwrColumn(nlev, fp64=False)
     xs, ys = [], []
@@ -183,5 +183,25 @@ def test_emulator_training_and_gain(gpu):
     print(f"emulator per-layer kernels {8192 / t_em_layers:.3e} col/s", flush=True)
     print(f"plain column {8192 / t_col:.3e} col/s, emulator {8192 / t_em:.3e} col/s, "
           f"gain x{t_col / t_em:.2f}", flush=True)
-    # the network learns the column: far better than predicting the mean
+    # the network learns the column: far better than predicting the mean,
+    # and within the 0.5 K/day RMS heating-rate bar
     assert rms < 0.5 * ref_rms
+    assert rms <= 0.5, rms
+    # what the trained network gains over the column it stands in for
+    gain = t_col / t_em
+    assert gain >= 20.0, gain
+    # columns outside the training envelope go to the plain column
+    out = columns_to_device(make_columns(512, nlev, 8), torch.float32)
+    out["dqco2"][:, :64] = 1e7  # 100x the largest CO2 of the training columns
+    h_g, o_g, n_plain = em.predict_guarded(out, col)
+    h_c, o_c = col(out)
+    h_e, o_e = em.predict(out)
+    torch.cuda.synchronize()
+    assert n_plain >= 64
+    assert torch.equal(h_g[:, :64], h_c[:, :64]) and torch.equal(o_g[:64], o_c[:64])
+    inside = torch.ones(512, dtype=torch.bool, device="cuda")
+    inside[:64] = False
+    # the envelope holds almost every column drawn like the training set
+    assert n_plain <= 64 + 512 // 20, n_plain
+    same = torch.all(h_g == h_e, dim=0) & (o_g == o_e)
+    assert bool(torch.all(same | ~inside | (h_g == h_c).all(dim=0)))
