@@ -10,10 +10,12 @@ The headline value is the split-BF16 (three-way split, FP32 accumulate, MFMA)
 path; the plain FP64 GPU path is timed beside it, and both are checked
 against the FP64 oracle on two members.
 
-The oracle (oracle/, test infrastructure) appears here in two roles only:
+The oracle (oracle/, test infrastructure) appears here in three roles only:
 as the checker of the outputs the GPU already produced (the error fields,
-computed on the host after every timed loop has ended), and as the timed
-CPU baseline (cpu_baseline, and the plain CPU column of the emulator leg).
+computed on the host after every timed loop has ended), as the timed CPU
+baseline (cpu_baseline, and the plain CPU column of the emulator leg), and
+as the source of the dynamics leg's vertical set-up (initsi's arrays, which
+the model computes once on the host before any transform).
 No timed GPU region calls it, and the transforms fail without the HIP
 library instead of falling back to it.
 
@@ -56,6 +58,8 @@ def parse():
                     help="keep the default tiles instead of timing the variants")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the T42 L20 headline (skip ensemble, T127, emulator)")
+    ap.add_argument("--extras", default="ensemble,t127,dynamics,emulator",
+                    help="comma-separated extra workloads to run beside the headline")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (the "
                          "reference kernel that scales the profiler's byte counters)")
@@ -246,6 +250,77 @@ def bench_t127(args, ora):
             "inverse_qi_mfma_TFLOPs": round(fl_qi / t_qi / 1e12, 1),
             "inverse_qi_mfma_frac_of_bf16_peak": round(fl_qi / t_qi / 2.5e15, 4),
             "grid_rel_l2_vs_fp64_oracle": eg, "spectral_rel_l2_vs_fp64_oracle": es_}
+
+
+def _mfma_flops_dirx(ntru, nlev, nmem, nslot):
+    """FP32 MFMA flops executed by one k_dirx_mf launch: per block (member,
+    8 levels, wavenumber range) and latitude-pair step of 4, one
+    16x16x4 product per (m, parity, non-empty row tile, table)"""
+    npair = {21: 32, 42: 64, 63: 96, 127: 192}[ntru] // 2
+    tiles = sum(((ntru - m - p) // 2 + 16) // 16 for m in range(ntru + 1) for p in (0, 1)
+                if ntru - m - p >= 0)
+    return nmem * ((nlev + 7) // 8) * tiles * nslot * (npair // 4) * 2 * 16 * 16 * 4
+
+
+def bench_dynamics(args, ora):
+    """The device-resident transform group of one dynamics step
+    (exo_gridpointa: sp2fc, sp2fcdmu, dv2uv, d/dlambda, fc2gp, calcgp,
+    gp2fc, fc2sp, mktend, qtend) at T42 L20 on 32 members, and its mktend
+    alone: the MFMA direct sums over three tables."""
+    import numpy as np
+    import torch
+
+    from exoplasim_amd import sht
+    from exoplasim_amd.dynamics import Dynamics
+    from exoplasim_amd.synth import spectral_fields
+
+    ntru, nlev, nmem, mode = 42, 20, 32, args.mode
+    ct = (6371220.0 * 7.292e-5) ** 2 / 287.0  # temperatures in units of ct
+    scal = (287.0 / 461.5, 1870.0 / (287.0 / 0.286) - 1.0, 101100.0, 7.292e-5)
+    v = ora.vertical(nlev, t0=np.linspace(220.0, 288.0, nlev) / ct)
+    plan = sht.Plan(ntru)
+    dyn = Dynamics.from_vertical(v, *scal)
+    nhor = plan.nlat * plan.nlon
+    nfl = 4 * nlev + 1
+    st0 = 0.01 * np.stack([spectral_fields(ntru, nfl, 5000 + k, 1.5, 0.0) for k in range(nmem)])
+    st0[:, :nlev, 2] += 1.63299310207  # planetary vorticity at Re(m=0, n=1)
+    st0[:, 3 * nlev:4 * nlev] *= 0.1    # humidity
+    st, work, tend = dyn.alloc_gridpointa(plan, nmem, mode)
+    st.copy_(torch.from_numpy(st0).to(st.dtype))
+    steps = max(5, args.steps // 2)
+    t = time_loop(lambda: dyn.gridpointa(plan, mode, st, work, tend, nqspec=1), steps, 2) / steps
+    # mktend alone on Fourier inputs of the same shape
+    dt = plan.dtype_of(mode)
+    ins = [torch.randn((nmem, nlev, plan.nlat, plan.nlon), device="cuda", dtype=dt)
+           for _ in range(6)]
+    outs = [torch.empty((nmem, nlev, plan.nrsp), device="cuda", dtype=dt) for _ in range(3)]
+    t_mk = time_loop(lambda: plan.mktend(mode, *outs, *ins, nlev), steps, 2) / steps
+    es = ins[0].element_size()
+    mk_bytes = nmem * nlev * (6 * nhor + 3 * plan.nrsp) * es
+    # useful flops: 16 real table terms (D 6, T 6, Z 4) per coefficient and pair
+    ncsp = plan.nrsp // 2
+    mk_useful = nmem * nlev * 16 * ncsp * (plan.nlat // 2) * 2
+    mk_exec = sum(_mfma_flops_dirx(ntru, nlev, nmem, ns) for ns in (3, 3, 2))
+    torch.cuda.synchronize()
+    got = tend[0].double().cpu().numpy()
+    want, _ = ora.gridpointa(ntru, nlev, st0[0].astype(np.float32).astype(np.float64)
+                             if mode != "f64" else st0[0], v, scal, 1)
+    errs = {}
+    for i, name in enumerate(("szt", "sdt", "stt", "sqt")):
+        part = slice(i * nlev, (i + 1) * nlev)
+        errs[name] = rel_l2(got[part], want[part])
+    errs["spt"] = rel_l2(got[4 * nlev], want[4 * nlev])
+    dyn.close()
+    plan.close()
+    return {"members": nmem, "mode": mode,
+            "gridpointa_ms_per_step": round(t * 1e3, 4),
+            "member_steps_per_s": round(nmem / t, 1),
+            "tendency_rel_l2_vs_fp64_oracle_member0": errs,
+            "mktend_ms": round(t_mk * 1e3, 4),
+            "mktend_GBs": round(mk_bytes / t_mk / 1e9, 1),
+            "mktend_useful_TFLOPs": round(mk_useful / t_mk / 1e12, 2),
+            "mktend_mfma_executed_TFLOPs": round(mk_exec / t_mk / 1e12, 2),
+            "mktend_mfma_frac_of_fp32_peak": round(mk_exec / t_mk / 157.3e12, 4)}
 
 
 def bench_emulator(args, ora):
@@ -459,8 +534,9 @@ def main():
         cpu = cpu_baseline(args.cpu_seconds, sp_host[0])
     extras = {}
     if not args.no_extras:
-        for name, fn in (("ensemble", bench_ensemble), ("t127", bench_t127),
-                         ("emulator", bench_emulator)):
+        legs = (("ensemble", bench_ensemble), ("t127", bench_t127),
+                ("dynamics", bench_dynamics), ("emulator", bench_emulator))
+        for name, fn in (leg for leg in legs if leg[0] in args.extras.split(",")):
             t0 = time.perf_counter()
             try:
                 extras[name] = fn(args, ora)

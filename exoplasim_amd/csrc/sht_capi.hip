@@ -53,6 +53,7 @@ static void free_tables(Tables& t) {
   for (int i = 0; i < NTAB; ++i) {
     if (t.inv_bf[i]) (void)hipFree(t.inv_bf[i]);
     if (t.inv_f32[i]) (void)hipFree(t.inv_f32[i]);
+    if (t.dir_f32[i]) (void)hipFree(t.dir_f32[i]);
   }
   t = Tables();
 }
@@ -109,6 +110,15 @@ static int build_tables(Tables& t, int ntru, const double* sid, const double* gw
     const long nth = nbf / 3;
     k_pack_inv_bf<<<(unsigned)((nth + 255) / 256), 256>>>(ntru, t.npair, tt, t.filt64,
                                                           t.inv_bf[tt]);
+    HIPCK(hipGetLastError());
+  }
+  t.dir_rb = ((ntru + 2) / 2 + 15) / 16;
+  const int dir_tabs[] = {TQC, TQE, TQM, TQQ};
+  for (int tt : dir_tabs) {
+    const long nd = (long)(ntru + 1) * 2 * t.dir_rb * t.npair * 16;
+    HIPCK(hipMalloc(&t.dir_f32[tt], sizeof(float) * nd));
+    k_pack_dir<<<(unsigned)((nd + 255) / 256), 256>>>(ntru, t.npair, t.dir_rb, tt, t.filt64,
+                                                      t.dir_f32[tt]);
     HIPCK(hipGetLastError());
   }
   k_twiddle<<<(t.nlon + 255) / 256, 256>>>(t.nlon, t.tw64, t.tw32);

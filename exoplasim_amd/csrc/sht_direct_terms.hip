@@ -7,7 +7,9 @@
 // where X_i is input i folded into E = N+S (class sym) / O = N-S (class anti)
 // and the class of table t at (m,n) is sym iff (n-m even) != (t is Q-type).
 // One launch computes one output field (re and im) for every level; the
-// caller launches once per output.
+// caller launches once per output.  FP64 runs k_dirx (VALU sums, one block
+// per level and latitude-pair block); the FP32 modes run k_dirx_mf (MFMA,
+// 8 levels per block).
 #include <type_traits>
 
 #include "sht_internal.h"
@@ -17,13 +19,11 @@ namespace exo {
 template <int MODE>
 using RealX = typename std::conditional<MODE == MODE_F64, double, float>::type;
 
-// latitude pairs per block so that 3 folded inputs fit in about 100 KB of LDS
+// latitude pairs per block so that 3 folded FP64 inputs fit in about
+// 100 KB of LDS
 template <int T, int MODE>
 struct DirxCfg {
-  static constexpr int PB = (T == 21) ? 16
-                          : (T == 42) ? (MODE == MODE_F64 ? 16 : 32)
-                          : (T == 63) ? (MODE == MODE_F64 ? 12 : 24)
-                                      : (MODE == MODE_F64 ? 8 : 16);
+  static constexpr int PB = (T == 21) ? 16 : (T == 42) ? 16 : (T == 63) ? 12 : 8;
   static constexpr int NPB = TS<T>::NPAIR / PB;
 };
 
@@ -84,6 +84,197 @@ __global__ __launch_bounds__(256) void k_dirx(DirxArgs a, const void* tabv, void
   }
 }
 
+// ---------------------------------------------------------------------------
+// the same sums on MFMA (FP32 modes)
+// ---------------------------------------------------------------------------
+// Per zonal wavenumber m and parity p of n - m, each output is a GEMM over
+// the latitude pairs:  C[n][lev, oc] = sum_t sum_pair A_t[n][pair] B_t[pair][lev, oc]
+// with A_t the table t at the rows n = m + p + 2j (packed by k_pack_dir) and
+// B_t[pair][lev, oc] = sign * X_in[cls][pair][m][lev][ic] for the term of
+// table t and output component oc.  One v_mfma_f32_16x16x4_f32 takes 16 rows
+// n x 4 pairs of A and 4 pairs x 16 columns (8 levels x re/im) of B.
+//
+// A block owns one member, 8 levels and a range of MR wavenumbers; it walks
+// the latitude pairs 4 at a time: the folded Fourier values of those pairs
+// (all inputs, its levels and wavenumbers) are staged in LDS, the next
+// pairs' values are requested while the MFMAs of the current ones run, and
+// every (m, p, row tile) accumulates in registers across all pairs, so the
+// sum over latitudes is complete in the block (no partial sums).  Wave w
+// takes m = m0 + w, m0 + w + 8, ...
+template <int T>
+struct DirxMfCfg {
+  static constexpr int MR = (T == 63) ? 32 : (T == 127) ? 16 : 22;  // wavenumbers per block
+  static constexpr int NMR = (TS<T>::NM + MR - 1) / MR;
+  static constexpr int NT = 512, NW = NT / 64;
+  static constexpr int MPW = (MR + NW - 1) / NW;                     // wavenumbers per wave
+  static constexpr int LG = 8;                                       // levels per block
+  static constexpr int KC = 4;                                       // pairs per K step
+  static constexpr int KS = LG * 2 * MR + 2;  // floats per (input, class, pair), padded
+};
+
+// tables per term list (mktend, qtend and uv2dv use at most three)
+constexpr int NSLOT = 3;
+
+// the term list regrouped by table: slot s multiplies table tab[s] by, for
+// output component oc, sign[s][oc] * X_in[s][oc] component ic[s][oc]
+struct DirxSlots {
+  const float* tab[NSLOT];
+  float sign[NSLOT][2];
+  int in[NSLOT][2], ic[NSLOT][2];
+  int isq[NSLOT];
+  int nslot;
+};
+
+template <int T>
+__global__ __launch_bounds__(512) void k_dirx_mf(DirxArgs a, DirxSlots sl) {
+  using S = TS<T>;
+  using C = DirxMfCfg<T>;
+  constexpr int N = S::NLON, NLAT = S::NLAT, NPAIR = S::NPAIR, RB = S::RB;
+  constexpr int MR = C::MR, LG = C::LG, KC = C::KC, KS = C::KS, MPW = C::MPW, NT = C::NT;
+  constexpr int NJ = 2 * MR;  // reals of one staged row segment
+  // staged values per K step: (input, level, pair, real) with the real fastest
+  constexpr int MAXIT = (3 * LG * KC * NJ + NT - 1) / NT;
+  __shared__ float X[3 * 2 * KC * KS];  // [input][class][pair][level][m][re/im]
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nlg = (a.nlev + LG - 1) / LG;
+  const int mr = blockIdx.x % C::NMR;
+  const int lg = (blockIdx.x / C::NMR) % nlg;
+  const int mem = blockIdx.x / (C::NMR * nlg);
+  if (mem >= a.nmem) return;
+  const int m0 = mr * MR, lev0 = lg * LG;
+  const int nitem = a.nin * LG * KC * NJ;
+  const int nj = (S::NM - m0 < MR ? S::NM - m0 : MR) * 2;  // reals below the truncation
+
+  float rn[MAXIT], rs[MAXIT];
+  auto load = [&](int c) {
+#pragma unroll
+    for (int it = 0; it < MAXIT; ++it) {
+      const int e = tid + NT * it;
+      const int j = e % NJ, k = (e / NJ) % KC, l = (e / (NJ * KC)) % LG, i = e / (NJ * KC * LG);
+      rn[it] = 0.f;
+      rs[it] = 0.f;
+      if (e < nitem && lev0 + l < a.nlev && j < nj) {
+        const float* src = (const float*)a.in[i] + mem * a.in_stride +
+                           (long)(lev0 + l) * NLAT * N + 2 * m0 + j;
+        const int pair = c * KC + k;
+        rn[it] = src[(long)pair * N];
+        rs[it] = src[(long)(NLAT - 1 - pair) * N];
+      }
+    }
+  };
+  // fold into E = N + S (class 0) and O = N - S (class 1)
+  auto store = [&]() {
+#pragma unroll
+    for (int it = 0; it < MAXIT; ++it) {
+      const int e = tid + NT * it;
+      if (e < nitem) {
+        const int j = e % NJ, k = (e / NJ) % KC, l = (e / (NJ * KC)) % LG, i = e / (NJ * KC * LG);
+        const int o = l * NJ + j;
+        X[((i * 2 + 0) * KC + k) * KS + o] = rn[it] + rs[it];
+        X[((i * 2 + 1) * KC + k) * KS + o] = rn[it] - rs[it];
+      }
+    }
+  };
+
+  // A operand: row r = lane & 15, pair c KC + (lane >> 4); B operand: column
+  // lane & 15 = (level l, component oc), pair lane >> 4
+  const int r = lane & 15, kq = lane >> 4, l = (lane & 15) >> 1, oc = lane & 1;
+  f32x4 acc[MPW][2][RB];
+#pragma unroll
+  for (int mi = 0; mi < MPW; ++mi)
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int rt = 0; rt < RB; ++rt) acc[mi][p][rt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // this lane's B source per slot: offset of (input, class 0, pair kq,
+  // level l, component ic) in X and the sign (0: no term for component oc)
+  int boff[NSLOT];
+  float bsg[NSLOT];
+#pragma unroll
+  for (int s = 0; s < NSLOT; ++s) {
+    boff[s] = ((oc ? sl.in[s][1] : sl.in[s][0]) * 2 * KC + kq) * KS + l * NJ +
+              (oc ? sl.ic[s][1] : sl.ic[s][0]);
+    bsg[s] = oc ? sl.sign[s][1] : sl.sign[s][0];
+  }
+  auto ntile = [](int m, int p) {
+    const int d = T - m - p;
+    return d < 0 ? 0 : (d / 2 + 16) / 16;
+  };
+
+  constexpr int NC = NPAIR / KC;
+  load(0);
+  for (int c = 0; c < NC; ++c) {
+    __syncthreads();  // the previous step's MFMAs have read X
+    store();
+    __syncthreads();
+    // every A operand of this step first (all row tiles: the packed table is
+    // zero past the truncation), so that their latencies overlap; then the
+    // next step's inputs, whose loads stay in flight through the MFMAs
+    float av[MPW][NSLOT][2][RB];
+#pragma unroll
+    for (int mi = 0; mi < MPW; ++mi) {
+      const int ml = wv + C::NW * mi, m = m0 + ml;
+      if (ml >= MR || m > T) continue;
+#pragma unroll
+      for (int s = 0; s < NSLOT; ++s) {
+        if (s >= sl.nslot) break;
+        const float* tp = sl.tab[s] + ((long)(m * 2) * RB * NPAIR + c * KC + kq) * 16 + r;
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+          for (int rt = 0; rt < RB; ++rt) av[mi][s][p][rt] = tp[((long)p * RB + rt) * NPAIR * 16];
+      }
+    }
+    if (c + 1 < NC) load(c + 1);
+#pragma unroll
+    for (int mi = 0; mi < MPW; ++mi) {
+      const int ml = wv + C::NW * mi, m = m0 + ml;
+      if (ml >= MR || m > T) continue;
+      const int nt0 = ntile(m, 0), nt1 = ntile(m, 1);
+#pragma unroll
+      for (int s = 0; s < NSLOT; ++s) {
+        if (s >= sl.nslot) break;
+        float b[2];
+#pragma unroll
+        for (int cls = 0; cls < 2; ++cls) {
+          const float x = X[boff[s] + cls * KC * KS + 2 * ml];
+          b[cls] = bsg[s] == 0.f ? 0.f : bsg[s] * x;
+        }
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          // class of the folded input: symmetric (E) iff (n - m even) != Q-type table
+          const float bv = b[(p == 0) == (sl.isq[s] != 0) ? 1 : 0];
+          const int nt = p ? nt1 : nt0;
+#pragma unroll
+          for (int rt = 0; rt < RB; ++rt)
+            if (rt < nt)
+              acc[mi][p][rt] =
+                  __builtin_amdgcn_mfma_f32_16x16x4f32(av[mi][s][p][rt], bv, acc[mi][p][rt], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // lane holds rows 4 kq + v of column (l, oc)
+  if (lev0 + l >= a.nlev) return;
+  float* out = (float*)a.out + mem * a.out_stride + (long)(lev0 + l) * S::NRSP;
+#pragma unroll
+  for (int mi = 0; mi < MPW; ++mi) {
+    const int ml = wv + C::NW * mi, m = m0 + ml;
+    if (ml >= MR || m > T) continue;
+    const int w0 = w0_of(T, m);
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int rt = 0; rt < RB; ++rt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int dn = p + 2 * (16 * rt + 4 * kq + v);
+          if (m + dn <= T) out[2 * (w0 + dn) + oc] = acc[mi][p][rt][v];
+        }
+  }
+}
+
 // sum of the latitude-pair blocks' partial sums in a fixed order; n reals
 // per block, per_mem of them per member (member stride out_stride in out)
 template <typename R>
@@ -95,6 +286,31 @@ __global__ void k_sum_blocks(const R* scratch, R* out, int npb, long n, long per
   for (int b = 0; b < npb; ++b) s += scratch[(long)b * n + i];
   const long mem = i / per_mem;
   out[mem * out_stride + (i - mem * per_mem)] = s;
+}
+
+template <int T>
+static int dirx_mf_t(const Tables& tb, const DirxArgs& a, hipStream_t s) {
+  using C = DirxMfCfg<T>;
+  if (a.nlev <= 0 || a.nmem <= 0) return 0;
+  DirxSlots sl{};
+  for (int t = 0; t < a.nterm; ++t) {
+    const DirxTerm& tm = a.terms[t];
+    int k = 0;
+    while (k < sl.nslot && sl.tab[k] != tb.dir_f32[tm.table]) ++k;
+    if (k == sl.nslot) {
+      if (k == NSLOT || !tb.dir_f32[tm.table]) return -2;
+      sl.tab[k] = tb.dir_f32[tm.table];
+      sl.isq[k] = table_is_q(tm.table) ? 1 : 0;
+      ++sl.nslot;
+    }
+    if (sl.sign[k][tm.oc] != 0.f) return -2;  // one term per (table, component)
+    sl.sign[k][tm.oc] = tm.sign > 0 ? 1.f : -1.f;
+    sl.in[k][tm.oc] = tm.in;
+    sl.ic[k][tm.oc] = tm.ic;
+  }
+  const long nlg = (a.nlev + C::LG - 1) / C::LG;
+  k_dirx_mf<T><<<(unsigned)((long)a.nmem * nlg * C::NMR), C::NT, 0, s>>>(a, sl);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 template <int T, int MODE>
@@ -115,31 +331,28 @@ static int dirx_t(const Tables& tb, const DirxArgs& a, void* scratch, size_t sby
 }
 
 size_t dirx_scratch_bytes(const Tables& tb, int mode, int nlev) {
+  if (mode != MODE_F64) return 0;  // the MFMA form sums all pairs in one block
   const long n = (long)nlev * 2 * tb.ncsp;
-  const size_t el = (mode == MODE_F64) ? 8 : 4;
   int npb = 1;
-  const bool d = (mode == MODE_F64);
   switch (tb.ntru) {
-    case 21: npb = d ? DirxCfg<21, MODE_F64>::NPB : DirxCfg<21, MODE_F32>::NPB; break;
-    case 42: npb = d ? DirxCfg<42, MODE_F64>::NPB : DirxCfg<42, MODE_F32>::NPB; break;
-    case 63: npb = d ? DirxCfg<63, MODE_F64>::NPB : DirxCfg<63, MODE_F32>::NPB; break;
-    case 127: npb = d ? DirxCfg<127, MODE_F64>::NPB : DirxCfg<127, MODE_F32>::NPB; break;
+    case 21: npb = DirxCfg<21, MODE_F64>::NPB; break;
+    case 42: npb = DirxCfg<42, MODE_F64>::NPB; break;
+    case 63: npb = DirxCfg<63, MODE_F64>::NPB; break;
+    case 127: npb = DirxCfg<127, MODE_F64>::NPB; break;
   }
-  return npb > 1 ? el * n * npb : 0;
+  return npb > 1 ? sizeof(double) * n * npb : 0;
 }
 
 int launch_dirx(const Tables& tb, int mode, const DirxArgs& a, void* scratch, size_t sbytes,
                 hipStream_t s) {
+  // FP64: VALU sums per level; FP32 modes: MFMA over 8 levels per block
   const bool d = (mode == MODE_F64);
   switch (tb.ntru) {
-    case 21: return d ? dirx_t<21, MODE_F64>(tb, a, scratch, sbytes, s)
-                      : dirx_t<21, MODE_F32>(tb, a, scratch, sbytes, s);
-    case 42: return d ? dirx_t<42, MODE_F64>(tb, a, scratch, sbytes, s)
-                      : dirx_t<42, MODE_F32>(tb, a, scratch, sbytes, s);
-    case 63: return d ? dirx_t<63, MODE_F64>(tb, a, scratch, sbytes, s)
-                      : dirx_t<63, MODE_F32>(tb, a, scratch, sbytes, s);
-    case 127: return d ? dirx_t<127, MODE_F64>(tb, a, scratch, sbytes, s)
-                       : dirx_t<127, MODE_F32>(tb, a, scratch, sbytes, s);
+    case 21: return d ? dirx_t<21, MODE_F64>(tb, a, scratch, sbytes, s) : dirx_mf_t<21>(tb, a, s);
+    case 42: return d ? dirx_t<42, MODE_F64>(tb, a, scratch, sbytes, s) : dirx_mf_t<42>(tb, a, s);
+    case 63: return d ? dirx_t<63, MODE_F64>(tb, a, scratch, sbytes, s) : dirx_mf_t<63>(tb, a, s);
+    case 127:
+      return d ? dirx_t<127, MODE_F64>(tb, a, scratch, sbytes, s) : dirx_mf_t<127>(tb, a, s);
     default: return -2;
   }
 }

@@ -211,6 +211,27 @@ __global__ void k_pack_inv_bf(int ntru, int npair, int t, const double* filt64, 
   o[32 * rw] = ll;
 }
 
+// ---- table operand of the MFMA direct transforms (k_dirx_mf): per (m,
+// parity p of n - m, row tile rt, latitude pair) the 16 rows n = m + p +
+// 2 (16 rt + r) of one table, [m][p][rt][pair][r], so that the 64 lanes of
+// one v_mfma_f32_16x16x4_f32 A operand (16 rows x 4 pairs) read 256
+// contiguous bytes; zero where n > NTRU ----
+__global__ void k_pack_dir(int ntru, int npair, int rb, int t, const double* filt64, float* out) {
+  const int ncsp = (ntru + 1) * (ntru + 2) / 2;
+  const long tsz = (long)npair * ncsp;
+  const long total = (long)(ntru + 1) * 2 * rb * npair * 16;
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int r = (int)(i % 16);
+  const int pair = (int)((i / 16) % npair);
+  const int rt = (int)((i / (16L * npair)) % rb);
+  const int p = (int)((i / (16L * npair * rb)) % 2);
+  const int m = (int)(i / (32L * npair * rb));
+  const int n = m + p + 2 * (16 * rt + r);
+  out[i] = (n <= ntru) ? (float)filt64[t * tsz + (long)pair * ncsp + w0_of(ntru, m) + (n - m)]
+                       : 0.0f;
+}
+
 // twiddles e^{+2 pi i k/N} in FP64 and FP32
 __global__ void k_twiddle(int n, double* tw64, float* tw32) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;

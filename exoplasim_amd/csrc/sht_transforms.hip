@@ -119,6 +119,25 @@ int launch_inverse_rt(const Tables& tb, int mode, const InvArgs& aq, const InvAr
 
 int launch_forward(const Tables& tb, int mode, bool fused, const FwdArgs& a, void* scratch,
                    size_t sbytes, hipStream_t s) {
+  if (!fused && mode != MODE_F64 && tb.ntru == 127) {
+    // fc2sp of Fourier input at T127 in the FP32 modes: the MFMA direct
+    // sums with the one table qc, each field taking the place of a level
+    // (1.4x faster than k_fwd there; k_fwd is 2.4x faster at T42, where a
+    // block's 8 latitude-pair steps leave the MFMA sums latency-bound)
+    DirxArgs d{};
+    const long fsz = (long)tb.nlat * tb.nlon;
+    d.in[0] = (const float*)a.in + a.src0 * fsz;
+    d.out = (float*)a.sp + (long)a.dst0 * 2 * tb.ncsp;
+    d.nin = 1;
+    d.nlev = a.nfield;
+    d.nmem = a.nmem;
+    d.in_stride = a.in_stride;
+    d.out_stride = a.sp_stride;
+    d.nterm = 2;
+    d.terms[0] = DirxTerm{0, 0, 0, +1, (int8_t)a.table};
+    d.terms[1] = DirxTerm{0, 1, 1, +1, (int8_t)a.table};
+    return launch_dirx(tb, MODE_F32, d, nullptr, 0, s);
+  }
   switch (mode) {
     case MODE_F64: return fwd_mode<MODE_F64>(tb, fused, a, scratch, sbytes, s);
     case MODE_F32: return fwd_mode<MODE_F32>(tb, fused, a, scratch, sbytes, s);
