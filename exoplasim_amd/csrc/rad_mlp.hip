@@ -349,26 +349,54 @@ __global__ __launch_bounds__(EM_NT) void k_emulate(exo_lw_net net, int ncol, con
     if (tid < EM_MT) sflag[tid] = 0;
     __syncthreads();
   }
-  // the weights of the first two layers are on their way while the features
-  // are computed; the third layer's while the second runs
-  V8 w1[EM_KIN / 32][2], w2[EM_NH / 32][2], w3[EM_NH / 32][2];
+  // every layer's weights are on their way while the features are computed
+  // (one block per CU at a model step's 8192 columns: nothing else would
+  // hide a later request's latency)
+  const int lane = tid & 63, wv = tid >> 6;
+  const int q = lane >> 4, c = lane & 15;
+  const int nt = wv & 1, mt = (wv >> 1) & 1;
+  V8 w1[EM_KIN / 32][2], w2[EM_NH / 32][2], w3[EM_NH / 32][2], w4[EM_NH / 32];
   em_load<EM_KIN>((const uint16_t*)net.w[0], w1);
   em_load<EM_NH>((const uint16_t*)net.w[1], w2);
-  // features (exoplasim_amd.radiation.features), column index fastest
-  for (int idx = tid; idx < EM_MT * EM_KIN; idx += EM_NT) {
+  em_load<EM_NH>((const uint16_t*)net.w[2], w3);
+  if (wv < 4) {
+    const uint16_t* W4 = (const uint16_t*)net.w[3] + (long)(nt * 16 + c) * EM_NH + q * 8;
+#pragma unroll
+    for (int k = 0; k < EM_NH / 32; ++k) w4[k].q = *(const uint4*)(W4 + 32 * k);
+  }
+  // features (exoplasim_amd.radiation.features), column index fastest: all
+  // of a thread's inputs are requested before any is used
+  constexpr int NIT = EM_MT * EM_KIN / EM_NT;
+  static_assert(EM_MT * EM_KIN % EM_NT == 0, "feature items per thread");
+  float raw[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int idx = tid + EM_NT * it;
     const int m = idx % EM_MT, f = idx / EM_MT;
     const int col = min(col0 + m, ncol - 1);
+    const float* src = (f < EM_NLEV)           ? dt + (long)f * ncol
+                       : (f < 2 * EM_NLEV)     ? dq + (long)(f - EM_NLEV) * ncol
+                       : (f == 2 * EM_NLEV)    ? dt + (long)EM_NLEV * ncol
+                       : (f == 2 * EM_NLEV + 1) ? dp
+                                                : co2;
+    raw[it] = (f < EM_NIN) ? src[col] : 0.f;
+  }
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int idx = tid + EM_NT * it;
+    const int m = idx % EM_MT, f = idx / EM_MT;
+    const float x = raw[it];
     float v = 0.f;
     if (f < EM_NLEV)
-      v = (dt[(long)f * ncol + col] - 250.0f) / 50.0f;
+      v = (x - 250.0f) / 50.0f;
     else if (f < 2 * EM_NLEV)
-      v = (logf(fmaxf(dq[(long)(f - EM_NLEV) * ncol + col], 1e-12f)) + 10.0f) / 3.5f;
+      v = (logf(fmaxf(x, 1e-12f)) + 10.0f) / 3.5f;
     else if (f == 2 * EM_NLEV)
-      v = (dt[(long)EM_NLEV * ncol + col] - 265.0f) / 40.0f;
+      v = (x - 265.0f) / 40.0f;
     else if (f == 2 * EM_NLEV + 1)
-      v = logf(dp[col] / 1e5f) / 1.3f;
+      v = logf(x / 1e5f) / 1.3f;
     else if (f == 2 * EM_NLEV + 2)
-      v = (logf(co2[col]) - 5.75f) / 3.3f;
+      v = (logf(x) - 5.75f) / 3.3f;
     sx[m * EM_LDX + f] = f2bf(v);
     if (GUARD && f < EM_NIN && !(v >= flo[f] && v <= fhi[f])) atomicOr(&sflag[m], 1);
   }
@@ -376,25 +404,19 @@ __global__ __launch_bounds__(EM_NT) void k_emulate(exo_lw_net net, int ncol, con
   if (GUARD && tid < EM_MT && col0 + tid < ncol) outside[col0 + tid] = sflag[tid];
   em_hidden<EM_KIN, EM_LDX>(sx, w1, net.b[0], sh0);
   __syncthreads();
-  em_load<EM_NH>((const uint16_t*)net.w[2], w3);
   em_hidden<EM_NH, EM_LDH>(sh0, w2, net.b[1], sh1);
   __syncthreads();
   em_hidden<EM_NH, EM_LDH>(sh1, w3, net.b[2], sh0);
   __syncthreads();
   // output layer, transposed: waves 0..3 take feature tile wv&1 (n) and
   // column tile wv>>1 (m)
-  const int lane = tid & 63, wv = tid >> 6;
   if (wv >= 4) return;
-  const int q = lane >> 4, c = lane & 15;
-  const int nt = wv & 1, mt = wv >> 1;
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  const uint16_t* W4 = (const uint16_t*)net.w[3] + (long)(nt * 16 + c) * EM_NH + q * 8;
 #pragma unroll
-  for (int k0 = 0; k0 < EM_NH; k0 += 32) {
-    V8 w, h;
-    w.q = *(const uint4*)(W4 + k0);
-    h.q = *(const uint4*)(sh0 + (mt * 16 + c) * EM_LDH + k0 + q * 8);
-    acc = mfma(w, h, acc);
+  for (int k = 0; k < EM_NH / 32; ++k) {
+    V8 h;
+    h.q = *(const uint4*)(sh0 + (mt * 16 + c) * EM_LDH + 32 * k + q * 8);
+    acc = mfma(w4[k], h, acc);
   }
   const int col = col0 + mt * 16 + c;
   if (col >= ncol) return;
