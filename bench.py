@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--members", type=int, default=128)
     ap.add_argument("--mode", default="bf16x3")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--settle", type=float, default=0.3,
+                    help="seconds of untimed round trips before the warm-up steps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default tiles instead of timing the variants")
@@ -118,6 +120,18 @@ def time_loop(fn, steps, warmup):
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / 1e3  # seconds
+
+
+def settle(fn, seconds):
+    """untimed calls of fn for at least `seconds` of wall time, so that the
+    GPU clocks reach their sustained level before a timed loop"""
+    import torch
+
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(10):
+            fn()
+        torch.cuda.synchronize()
 
 
 def cpu_baseline(seconds: float, sp_member):
@@ -469,6 +483,10 @@ def main():
         def rt():
             plan.roundtrip(mode, sp_in, gp, sp_out, NLEV)
 
+        # the same K steps from cold, reported beside the value: the clocks
+        # are still rising during the first tens of milliseconds of work
+        t_cold = time_loop(rt, args.steps, args.warmup)
+        settle(rt, args.settle)
         t = time_loop(rt, args.steps, args.warmup)
         # per-call timing of the two halves of a round trip
         legs = {
@@ -488,6 +506,7 @@ def main():
         torch.cuda.synchronize()
         results[mode] = {
             "time": t,
+            "cold_ms_per_step": t_cold / args.steps * 1e3,
             "rt_per_s": nmem * args.steps / t,
             "ms_per_step": t / args.steps * 1e3,
             "kernels": kt,
@@ -552,6 +571,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(head["ms_per_step"], 4),
+        "cold_ms_per_step": round(head["cold_ms_per_step"], 4),
+        "settle_s": args.settle,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
