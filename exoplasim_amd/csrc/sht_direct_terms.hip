@@ -95,28 +95,29 @@ __global__ __launch_bounds__(256) void k_dirx(DirxArgs a, const void* tabv, void
 // n x 4 pairs of A and 4 pairs x 16 columns (8 levels x re/im) of B.
 //
 // A block owns one member, 8 levels and a range of MR wavenumbers; it walks
-// the latitude pairs 4 at a time: the folded Fourier values of those pairs
-// (all inputs, its levels and wavenumbers) are staged in LDS, the next
-// pairs' values are requested while the MFMAs of the current ones run, and
-// every (m, p, row tile) accumulates in registers across all pairs, so the
-// sum over latitudes is complete in the block (no partial sums).  Wave w
-// takes m = m0 + w, m0 + w + 8, ...
-template <int T>
+// the latitude pairs KC at a time: the folded Fourier values of those pairs
+// (all inputs, its levels and wavenumbers) are staged in LDS, every table
+// operand of the step is requested, then the next pairs' values, and the
+// MFMAs run while those are in flight; every (m, p, row tile) accumulates in
+// registers across all pairs, so the sum over latitudes is complete in the
+// block (no partial sums).  Wave w takes m = m0 + w, m0 + w + 8, ...  KC is
+// as large as the step's table operands allow in registers: fewer inputs
+// (tables), more pairs per step.
+template <int T, int NIN>
 struct DirxMfCfg {
   static constexpr int MR = (T == 63) ? 32 : (T == 127) ? 16 : 22;  // wavenumbers per block
   static constexpr int NMR = (TS<T>::NM + MR - 1) / MR;
   static constexpr int NT = 512, NW = NT / 64;
   static constexpr int MPW = (MR + NW - 1) / NW;                     // wavenumbers per wave
   static constexpr int LG = 8;                                       // levels per block
-  static constexpr int KC = 4;                                       // pairs per K step
+  static constexpr int KC = (NIN == 1) ? 16 : (NIN == 2) ? ((T == 63) ? 4 : 8) : 4;  // pairs per step
   static constexpr int KS = LG * 2 * MR + 2;  // floats per (input, class, pair), padded
 };
 
-// tables per term list (mktend, qtend and uv2dv use at most three)
-constexpr int NSLOT = 3;
-
 // the term list regrouped by table: slot s multiplies table tab[s] by, for
-// output component oc, sign[s][oc] * X_in[s][oc] component ic[s][oc]
+// output component oc, sign[s][oc] * X_in[s][oc] component ic[s][oc]; every
+// term list here has one table per input (mktend, qtend, uv2dv, fc2sp)
+constexpr int NSLOT = 3;
 struct DirxSlots {
   const float* tab[NSLOT];
   float sign[NSLOT][2];
@@ -125,16 +126,19 @@ struct DirxSlots {
   int nslot;
 };
 
-template <int T>
+template <int T, int NIN>
 __global__ __launch_bounds__(512) void k_dirx_mf(DirxArgs a, DirxSlots sl) {
   using S = TS<T>;
-  using C = DirxMfCfg<T>;
+  using C = DirxMfCfg<T, NIN>;
   constexpr int N = S::NLON, NLAT = S::NLAT, NPAIR = S::NPAIR, RB = S::RB;
   constexpr int MR = C::MR, LG = C::LG, KC = C::KC, KS = C::KS, MPW = C::MPW, NT = C::NT;
-  constexpr int NJ = 2 * MR;  // reals of one staged row segment
-  // staged values per K step: (input, level, pair, real) with the real fastest
-  constexpr int MAXIT = (3 * LG * KC * NJ + NT - 1) / NT;
-  __shared__ float X[3 * 2 * KC * KS];  // [input][class][pair][level][m][re/im]
+  constexpr int NKS = KC / 4;  // MFMA K steps (4 pairs each) per staged step
+  constexpr int NJ = 2 * MR;   // reals of one staged row segment
+  static_assert(NPAIR % KC == 0, "pairs per step");
+  // staged values per step: (input, level, pair, real) with the real fastest
+  constexpr int NITEM = NIN * LG * KC * NJ;
+  constexpr int MAXIT = (NITEM + NT - 1) / NT;
+  __shared__ float X[NIN * 2 * KC * KS];  // [input][class][pair][level][m][re/im]
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nlg = (a.nlev + LG - 1) / LG;
@@ -143,7 +147,6 @@ __global__ __launch_bounds__(512) void k_dirx_mf(DirxArgs a, DirxSlots sl) {
   const int mem = blockIdx.x / (C::NMR * nlg);
   if (mem >= a.nmem) return;
   const int m0 = mr * MR, lev0 = lg * LG;
-  const int nitem = a.nin * LG * KC * NJ;
   const int nj = (S::NM - m0 < MR ? S::NM - m0 : MR) * 2;  // reals below the truncation
 
   float rn[MAXIT], rs[MAXIT];
@@ -154,7 +157,7 @@ __global__ __launch_bounds__(512) void k_dirx_mf(DirxArgs a, DirxSlots sl) {
       const int j = e % NJ, k = (e / NJ) % KC, l = (e / (NJ * KC)) % LG, i = e / (NJ * KC * LG);
       rn[it] = 0.f;
       rs[it] = 0.f;
-      if (e < nitem && lev0 + l < a.nlev && j < nj) {
+      if (e < NITEM && lev0 + l < a.nlev && j < nj) {
         const float* src = (const float*)a.in[i] + mem * a.in_stride +
                            (long)(lev0 + l) * NLAT * N + 2 * m0 + j;
         const int pair = c * KC + k;
@@ -168,7 +171,7 @@ __global__ __launch_bounds__(512) void k_dirx_mf(DirxArgs a, DirxSlots sl) {
 #pragma unroll
     for (int it = 0; it < MAXIT; ++it) {
       const int e = tid + NT * it;
-      if (e < nitem) {
+      if (e < NITEM) {
         const int j = e % NJ, k = (e / NJ) % KC, l = (e / (NJ * KC)) % LG, i = e / (NJ * KC * LG);
         const int o = l * NJ + j;
         X[((i * 2 + 0) * KC + k) * KS + o] = rn[it] + rs[it];
@@ -177,8 +180,8 @@ __global__ __launch_bounds__(512) void k_dirx_mf(DirxArgs a, DirxSlots sl) {
     }
   };
 
-  // A operand: row r = lane & 15, pair c KC + (lane >> 4); B operand: column
-  // lane & 15 = (level l, component oc), pair lane >> 4
+  // A operand: row r = lane & 15, pair 4 ks + (lane >> 4) of the step; B
+  // operand: column lane & 15 = (level l, component oc), the same pair
   const int r = lane & 15, kq = lane >> 4, l = (lane & 15) >> 1, oc = lane & 1;
   f32x4 acc[MPW][2][RB];
 #pragma unroll
@@ -189,10 +192,10 @@ __global__ __launch_bounds__(512) void k_dirx_mf(DirxArgs a, DirxSlots sl) {
       for (int rt = 0; rt < RB; ++rt) acc[mi][p][rt] = f32x4{0.f, 0.f, 0.f, 0.f};
   // this lane's B source per slot: offset of (input, class 0, pair kq,
   // level l, component ic) in X and the sign (0: no term for component oc)
-  int boff[NSLOT];
-  float bsg[NSLOT];
+  int boff[NIN];
+  float bsg[NIN];
 #pragma unroll
-  for (int s = 0; s < NSLOT; ++s) {
+  for (int s = 0; s < NIN; ++s) {
     boff[s] = ((oc ? sl.in[s][1] : sl.in[s][0]) * 2 * KC + kq) * KS + l * NJ +
               (oc ? sl.ic[s][1] : sl.ic[s][0]);
     bsg[s] = oc ? sl.sign[s][1] : sl.sign[s][0];
@@ -211,19 +214,21 @@ __global__ __launch_bounds__(512) void k_dirx_mf(DirxArgs a, DirxSlots sl) {
     // every A operand of this step first (all row tiles: the packed table is
     // zero past the truncation), so that their latencies overlap; then the
     // next step's inputs, whose loads stay in flight through the MFMAs
-    float av[MPW][NSLOT][2][RB];
+    float av[MPW][NIN][2][RB][NKS];
 #pragma unroll
     for (int mi = 0; mi < MPW; ++mi) {
       const int ml = wv + C::NW * mi, m = m0 + ml;
       if (ml >= MR || m > T) continue;
 #pragma unroll
-      for (int s = 0; s < NSLOT; ++s) {
-        if (s >= sl.nslot) break;
+      for (int s = 0; s < NIN; ++s) {
         const float* tp = sl.tab[s] + ((long)(m * 2) * RB * NPAIR + c * KC + kq) * 16 + r;
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
-          for (int rt = 0; rt < RB; ++rt) av[mi][s][p][rt] = tp[((long)p * RB + rt) * NPAIR * 16];
+          for (int rt = 0; rt < RB; ++rt)
+#pragma unroll
+            for (int ks = 0; ks < NKS; ++ks)
+              av[mi][s][p][rt][ks] = tp[(((long)p * RB + rt) * NPAIR + 4 * ks) * 16];
       }
     }
     if (c + 1 < NC) load(c + 1);
@@ -233,24 +238,26 @@ __global__ __launch_bounds__(512) void k_dirx_mf(DirxArgs a, DirxSlots sl) {
       if (ml >= MR || m > T) continue;
       const int nt0 = ntile(m, 0), nt1 = ntile(m, 1);
 #pragma unroll
-      for (int s = 0; s < NSLOT; ++s) {
-        if (s >= sl.nslot) break;
-        float b[2];
+      for (int s = 0; s < NIN; ++s) {
 #pragma unroll
-        for (int cls = 0; cls < 2; ++cls) {
-          const float x = X[boff[s] + cls * KC * KS + 2 * ml];
-          b[cls] = bsg[s] == 0.f ? 0.f : bsg[s] * x;
-        }
+        for (int ks = 0; ks < NKS; ++ks) {
+          float b[2];
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
-          // class of the folded input: symmetric (E) iff (n - m even) != Q-type table
-          const float bv = b[(p == 0) == (sl.isq[s] != 0) ? 1 : 0];
-          const int nt = p ? nt1 : nt0;
+          for (int cls = 0; cls < 2; ++cls) {
+            const float x = X[boff[s] + (cls * KC + 4 * ks) * KS + 2 * ml];
+            b[cls] = bsg[s] == 0.f ? 0.f : bsg[s] * x;
+          }
 #pragma unroll
-          for (int rt = 0; rt < RB; ++rt)
-            if (rt < nt)
-              acc[mi][p][rt] =
-                  __builtin_amdgcn_mfma_f32_16x16x4f32(av[mi][s][p][rt], bv, acc[mi][p][rt], 0, 0, 0);
+          for (int p = 0; p < 2; ++p) {
+            // class of the folded input: symmetric (E) iff (n - m even) != Q-type table
+            const float bv = b[(p == 0) == (sl.isq[s] != 0) ? 1 : 0];
+            const int nt = p ? nt1 : nt0;
+#pragma unroll
+            for (int rt = 0; rt < RB; ++rt)
+              if (rt < nt)
+                acc[mi][p][rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[mi][s][p][rt][ks], bv,
+                                                                      acc[mi][p][rt], 0, 0, 0);
+          }
         }
       }
     }
@@ -290,7 +297,6 @@ __global__ void k_sum_blocks(const R* scratch, R* out, int npb, long n, long per
 
 template <int T>
 static int dirx_mf_t(const Tables& tb, const DirxArgs& a, hipStream_t s) {
-  using C = DirxMfCfg<T>;
   if (a.nlev <= 0 || a.nmem <= 0) return 0;
   DirxSlots sl{};
   for (int t = 0; t < a.nterm; ++t) {
@@ -308,8 +314,17 @@ static int dirx_mf_t(const Tables& tb, const DirxArgs& a, hipStream_t s) {
     sl.in[k][tm.oc] = tm.in;
     sl.ic[k][tm.oc] = tm.ic;
   }
-  const long nlg = (a.nlev + C::LG - 1) / C::LG;
-  k_dirx_mf<T><<<(unsigned)((long)a.nmem * nlg * C::NMR), C::NT, 0, s>>>(a, sl);
+  if (sl.nslot != a.nin) return -2;  // one table per input
+  const long nlg = (a.nlev + 7) / 8;
+  auto launch = [&](auto kern, int nmr) {
+    kern<<<(unsigned)((long)a.nmem * nlg * nmr), 512, 0, s>>>(a, sl);
+  };
+  switch (a.nin) {
+    case 1: launch(k_dirx_mf<T, 1>, DirxMfCfg<T, 1>::NMR); break;
+    case 2: launch(k_dirx_mf<T, 2>, DirxMfCfg<T, 2>::NMR); break;
+    case 3: launch(k_dirx_mf<T, 3>, DirxMfCfg<T, 3>::NMR); break;
+    default: return -2;
+  }
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

@@ -121,9 +121,15 @@ int launch_inverse_rt(const Tables& T, int mode, const InvArgs& aq, const InvArg
 int launch_forward(const Tables& T, int mode, bool fused, const FwdArgs& a, void* scratch,
                    size_t scratch_bytes, hipStream_t s);
 size_t forward_scratch_bytes(const Tables& T, int mode, int nmem, int nfield);
-// in-place FFTs of lot rows of n values, for nmem members mstride values apart
+// Fourier rows of the T127 FP32 direct transform of grid input
+size_t fwd_fft_scratch_bytes(const Tables& T, int nmem, int nfield);
+// in-place FFTs of lot rows of n values, for nmem members mstride values
+// apart; the forward FFT (dir -1) may read its rows from src instead
+// (smstride per member) and write only the first kmax complex coefficients
+// of each row (0: all n/2)
 int launch_fft(const Tables& T, int mode, int dir, void* a, int n, long lot, hipStream_t s,
-               int nmem = 1, long mstride = 0);
+               int nmem = 1, long mstride = 0, const void* src = nullptr, long smstride = 0,
+               int kmax = 0);
 // pre-split copy (PK layout, sht_common.h) of fp32 spectral fields [f0, f0+nf)
 // of nmem members for the split-BF16 inverse; out holds nmem * nf *
 // pk_field(ntru) bf16 values

@@ -28,11 +28,15 @@ namespace exo {
 // DIR +1: Fourier -> grid (fc2gp), DIR -1: grid -> Fourier (gp2fc)
 // ---------------------------------------------------------------------------
 // blockIdx.y selects a member: its rows start mstride values after the
-// previous member's
+// previous member's.  DIR -1 reads the rows from src (smstride per member;
+// src = a for the in-place transform) and writes the kmax first complex
+// coefficients of each row (kmax = N/2: all of them).
 template <int N, typename R, int DIR>
 __global__ __launch_bounds__(256) void k_fft_rows(R* a, long lot, const C2<R>* tw,
-                                                  long mstride) {
+                                                  long mstride, const R* src, long smstride,
+                                                  int kmax) {
   a += blockIdx.y * mstride;
+  src += blockIdx.y * smstride;
   constexpr int NF = FftBatch<N>::NF;
   __shared__ C2<R> fbuf[fft_buf_elems(NF * N)];
   __shared__ C2<R> tws[N];
@@ -70,16 +74,20 @@ __global__ __launch_bounds__(256) void k_fft_rows(R* a, long lot, const C2<R>* t
     };
     fft_run<N, +1, R>(fbuf, tws, nf, load, store);
   } else {
+    auto srow = [&](int f, int h) -> const R* {
+      const long r = r0 + 2 * f + h;
+      return r < lot ? src + r * N : nullptr;
+    };
     auto load = [&](int f, int k) -> C2<R> {
-      const R* x0 = row(f, 0);
-      const R* x1 = row(f, 1);
+      const R* x0 = srow(f, 0);
+      const R* x1 = srow(f, 1);
       return C2<R>{x0 ? x0[k] : R(0), x1 ? x1[k] : R(0)};
     };
     auto store = [&](int f, int k, C2<R> v) { fbuf[bix(f * N + k)] = v; };
     fft_run<N, -1, R>(fbuf, tws, nf, load, store);
     const R sc = R(0.5) / R(N);
-    for (int idx = threadIdx.x; idx < nf * (N / 2); idx += 256) {
-      const int f = idx / (N / 2), k = idx % (N / 2);
+    for (int idx = threadIdx.x; idx < nf * kmax; idx += 256) {
+      const int f = idx / kmax, k = idx % kmax;
       const C2<R> zk = fbuf[bix(f * N + k)];
       const C2<R> zn = fbuf[bix(f * N + ((N - k) % N))];
       R* x0 = row(f, 0);
@@ -119,19 +127,32 @@ int launch_inverse_rt(const Tables& tb, int mode, const InvArgs& aq, const InvAr
 
 int launch_forward(const Tables& tb, int mode, bool fused, const FwdArgs& a, void* scratch,
                    size_t sbytes, hipStream_t s) {
-  if (!fused && mode != MODE_F64 && tb.ntru == 127) {
-    // fc2sp of Fourier input at T127 in the FP32 modes: the MFMA direct
-    // sums with the one table qc, each field taking the place of a level
-    // (1.4x faster than k_fwd there; k_fwd is 2.4x faster at T42, where a
-    // block's 8 latitude-pair steps leave the MFMA sums latency-bound)
-    DirxArgs d{};
+  if (mode != MODE_F64 && tb.ntru == 127) {
+    // T127 in the FP32 modes: the MFMA direct sums with the one table qc,
+    // each field taking the place of a level; grid input first goes through
+    // the row FFT into scratch (coefficients m <= T only).  The fused
+    // k_fwd<127> holds one field-level per block and writes three
+    // latitude-partial sums (564 us against 431 us for 121 fields x 8
+    // members); at T21-T63 k_fwd is as fast or faster.
     const long fsz = (long)tb.nlat * tb.nlon;
-    d.in[0] = (const float*)a.in + a.src0 * fsz;
+    const void* fin = (const float*)a.in + a.src0 * fsz;
+    long fstride = a.in_stride;
+    if (fused) {
+      if (!scratch || sbytes < fwd_fft_scratch_bytes(tb, a.nmem, a.nfield)) return -4;
+      const long ms = (long)a.nfield * fsz;
+      const int rc = launch_fft(tb, mode, -1, scratch, tb.nlon, (long)a.nfield * tb.nlat, s,
+                                a.nmem, ms, fin, a.in_stride, tb.ntru + 1);
+      if (rc) return rc;
+      fin = scratch;
+      fstride = ms;
+    }
+    DirxArgs d{};
+    d.in[0] = fin;
     d.out = (float*)a.sp + (long)a.dst0 * 2 * tb.ncsp;
     d.nin = 1;
     d.nlev = a.nfield;
     d.nmem = a.nmem;
-    d.in_stride = a.in_stride;
+    d.in_stride = fstride;
     d.out_stride = a.sp_stride;
     d.nterm = 2;
     d.terms[0] = DirxTerm{0, 0, 0, +1, (int8_t)a.table};
@@ -147,7 +168,12 @@ int launch_forward(const Tables& tb, int mode, bool fused, const FwdArgs& a, voi
   }
 }
 
+size_t fwd_fft_scratch_bytes(const Tables& tb, int nmem, int nfield) {
+  return sizeof(float) * (size_t)nmem * nfield * tb.nlat * tb.nlon;
+}
+
 size_t forward_scratch_bytes(const Tables& tb, int mode, int nmem, int nfield) {
+  if (mode != MODE_F64 && tb.ntru == 127) return fwd_fft_scratch_bytes(tb, nmem, nfield);
   switch (mode) {
     case MODE_F64: return scr_mode<MODE_F64>(tb, nmem, nfield);
     case MODE_F32: return scr_mode<MODE_F32>(tb, nmem, nfield);
@@ -158,15 +184,19 @@ size_t forward_scratch_bytes(const Tables& tb, int mode, int nmem, int nfield) {
 }
 
 template <int N, typename R>
-static int fft_t(const void* tw, int dir, R* a, long lot, int nmem, long mstride, hipStream_t s) {
+static int fft_t(const void* tw, int dir, R* a, long lot, int nmem, long mstride, hipStream_t s,
+                 const R* src, long smstride, int kmax) {
   constexpr int NF = FftBatch<N>::NF;
   const long nblk = (lot + 2 * NF - 1) / (2 * NF);
   if (nblk <= 0 || nmem <= 0) return 0;
   const dim3 grid((unsigned)nblk, (unsigned)nmem);
   if (dir > 0)
-    k_fft_rows<N, R, +1><<<grid, 256, 0, s>>>(a, lot, (const C2<R>*)tw, mstride);
+    k_fft_rows<N, R, +1><<<grid, 256, 0, s>>>(a, lot, (const C2<R>*)tw, mstride, a, mstride,
+                                               N / 2);
   else
-    k_fft_rows<N, R, -1><<<grid, 256, 0, s>>>(a, lot, (const C2<R>*)tw, mstride);
+    k_fft_rows<N, R, -1><<<grid, 256, 0, s>>>(a, lot, (const C2<R>*)tw, mstride,
+                                               src ? src : a, src ? smstride : mstride,
+                                               kmax > 0 && kmax < N / 2 ? kmax : N / 2);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
@@ -234,20 +264,23 @@ int launch_pack(const Tables& tb, const float* sp, long sp_stride, int f0, int n
 }
 
 int launch_fft(const Tables& tb, int mode, int dir, void* a, int n, long lot, hipStream_t s,
-               int nmem, long mstride) {
+               int nmem, long mstride, const void* src, long smstride, int kmax) {
   if (n != tb.nlon) return -2;
   const bool d = (mode == MODE_F64);
+#define EXO_FFT_CASE(NN)                                                                    \
+  case NN:                                                                                  \
+    return d ? fft_t<NN, double>(tb.tw64, dir, (double*)a, lot, nmem, mstride, s,           \
+                                 (const double*)src, smstride, kmax)                        \
+             : fft_t<NN, float>(tb.tw32, dir, (float*)a, lot, nmem, mstride, s,             \
+                                (const float*)src, smstride, kmax);
   switch (n) {
-    case 64: return d ? fft_t<64, double>(tb.tw64, dir, (double*)a, lot, nmem, mstride, s)
-                      : fft_t<64, float>(tb.tw32, dir, (float*)a, lot, nmem, mstride, s);
-    case 128: return d ? fft_t<128, double>(tb.tw64, dir, (double*)a, lot, nmem, mstride, s)
-                       : fft_t<128, float>(tb.tw32, dir, (float*)a, lot, nmem, mstride, s);
-    case 192: return d ? fft_t<192, double>(tb.tw64, dir, (double*)a, lot, nmem, mstride, s)
-                       : fft_t<192, float>(tb.tw32, dir, (float*)a, lot, nmem, mstride, s);
-    case 384: return d ? fft_t<384, double>(tb.tw64, dir, (double*)a, lot, nmem, mstride, s)
-                       : fft_t<384, float>(tb.tw32, dir, (float*)a, lot, nmem, mstride, s);
+    EXO_FFT_CASE(64)
+    EXO_FFT_CASE(128)
+    EXO_FFT_CASE(192)
+    EXO_FFT_CASE(384)
     default: return -2;
   }
+#undef EXO_FFT_CASE
 }
 
 }  // namespace exo
