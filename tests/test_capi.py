@@ -129,3 +129,41 @@ def test_dropin_size_checks():
             sht.dv2uv(short, np.zeros((10, 2 * 253)))
     finally:
         sht._state.update(saved)
+
+
+def test_new_entry_points_argument_errors():
+    """the multi-process, output, dynamics and guarded-emulator entry points
+    reject bad arguments, and accept empty work, before any device work"""
+    import ctypes
+
+    from exoplasim_amd import dynamics, multiproc, output
+
+    mp, out, dyn = multiproc._lib(), output._lib(), dynamics._lib()
+    ERR = -1
+    # mpsumsc: no processes; no fields is nothing to do
+    assert mp.exo_sht_sumsc(0, 0, 4, 1, None, None, None) == ERR
+    assert mp.exo_sht_sumsc(0, 2, 4, 0, None, None, None) == 0
+    assert mp.exo_sht_sumsc(0, 2, 4, 1, None, None, None) == ERR
+    assert mp.exo_sht_sumsc(7, 2, 4, 1, None, None, None) == ERR  # mode
+    # mrtrunc / mrexpand: the low truncation above the high one
+    assert mp.exo_sht_mrtrunc(0, 21, 42, 1, None, 0, None, 0, None) == ERR
+    assert mp.exo_sht_mrexpand(0, 42, 21, 1, None, 0, None, 0, None) == ERR
+    assert mp.exo_sht_mrtrunc(0, 42, 21, 0, None, 0, None, 0, None) == 0
+    # partial fc2sp without a plan
+    assert mp.exo_sht_fc2sp_part(None, 0, 1, None, 0, 0, 0, 8, None, 0, 0, 506, 1, None) == ERR
+    # writesp: mode and truncation checks, empty work
+    assert out.exo_sht_writesp(9, 21, 1, None, 0, 1.0, None, None, 0, None) == ERR
+    assert out.exo_sht_writesp(0, -1, 1, None, 0, 1.0, None, None, 0, None) == ERR
+    assert out.exo_sht_writesp(0, 21, 0, None, 0, 1.0, None, None, 0, None) == 0
+    assert out.exo_sht_writesp(0, 21, 1, None, 0, 1.0, None, None, 0, None) == ERR
+    # dynamics: no handle, too few levels; the work-array size of gridpointa
+    h = ctypes.c_void_p()
+    d = (ctypes.c_double * 4)()
+    assert dyn.exo_dyn_create(None, 4, d, d, d, d, d, d, d, d, 0.286, 0.6, 0.6, 1e5,
+                              7.292e-5) == ERR
+    assert dyn.exo_dyn_create(ctypes.byref(h), 1, d, d, d, d, d, d, d, d, 0.286, 0.6, 0.6,
+                              1e5, 7.292e-5) == ERR
+    assert dyn.exo_dyn_destroy(None) == ERR
+    assert dyn.exo_gridpointa_fields(20) == 18 * 20 + 5
+    assert dyn.exo_gridpointa_fields(1) == ERR
+    assert dyn.exo_gridpointa(None, None, 0, 1, 1, None, 0, None, 0, None, 0, None) == ERR
