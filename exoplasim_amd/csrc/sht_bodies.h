@@ -19,14 +19,16 @@ using RealT = typename std::conditional<MODE == MODE_F64, double, float>::type;
 // V selects the tile variant the autotuner times: V = 1 halves the fields
 // per block (half the LDS, twice the blocks in flight); V = 2 keeps the
 // fields and runs the block with 512 threads (8 waves: twice the waves in
-// flight per CU for the same LDS, FFT batches of twice the rows).
+// flight per CU for the same LDS, FFT batches of twice the rows); V = 3
+// halves the fields and runs 512 threads (the persistent ensemble kernels,
+// whose LDS is sized for their largest truncation).
 template <int T, int MODE, int V = 0>
 struct InvCfg {
   static constexpr int LP = 8;  // latitude pairs per tile (= 16 MFMA rows)
   static constexpr int FB0 =
       (T == 127) ? (MODE == MODE_F64 ? 2 : 4) : (MODE == MODE_F64 ? 4 : 8);
-  static constexpr int FB = (V == 1) ? ((FB0 >> 1) < 2 ? 2 : (FB0 >> 1)) : FB0;
-  static constexpr int NT = (V == 2) ? 512 : 256;  // threads per block
+  static constexpr int FB = (V == 1 || V == 3) ? ((FB0 >> 1) < 2 ? 2 : (FB0 >> 1)) : FB0;
+  static constexpr int NT = (V == 2 || V == 3) ? 512 : 256;  // threads per block
 };
 
 template <int T, int MODE, int V = 0>
@@ -36,9 +38,9 @@ struct FwdCfg {
                           : (T == 63) ? (MODE == MODE_F64 ? 24 : 48)
                                       : (MODE == MODE_F64 ? 16 : 32);
   static constexpr int FBF0 = (T == 21) ? 4 : (T == 42) ? (MODE == MODE_F64 ? 1 : 2) : 1;
-  static constexpr int FBF = (V == 1) ? ((FBF0 >> 1) < 1 ? 1 : (FBF0 >> 1)) : FBF0;
+  static constexpr int FBF = (V == 1 || V == 3) ? ((FBF0 >> 1) < 1 ? 1 : (FBF0 >> 1)) : FBF0;
   static constexpr int NPB = TS<T>::NPAIR / PB;
-  static constexpr int NT = (V == 2) ? 512 : 256;
+  static constexpr int NT = (V == 2 || V == 3) ? 512 : 256;
 };
 
 template <typename R>

@@ -70,8 +70,24 @@ __device__ __forceinline__ int next_item(int* counter, int* s_item) {
   return item;
 }
 
-template <int T, int MODE>
+template <int T, int MODE, int NT>
 __device__ __forceinline__ void ens_inv_item(const EnsInvJob& jb, int blk, char* lds) {
+  if (NT == 512) {
+    if (jb.a.kind == INV_UV) {
+      if (jb.var == 3)
+        inv_body<T, MODE, INV_UV, true, 3>(jb.a, blk, jb.A0, jb.A1, jb.plane, jb.tv, jb.tw, lds);
+      else
+        inv_body<T, MODE, INV_UV, true, 2>(jb.a, blk, jb.A0, jb.A1, jb.plane, jb.tv, jb.tw, lds);
+    } else {
+      if (jb.var == 3)
+        inv_body<T, MODE, INV_SCALAR, true, 3>(jb.a, blk, jb.A0, jb.A1, jb.plane, jb.tv, jb.tw,
+                                               lds);
+      else
+        inv_body<T, MODE, INV_SCALAR, true, 2>(jb.a, blk, jb.A0, jb.A1, jb.plane, jb.tv, jb.tw,
+                                               lds);
+    }
+    return;
+  }
   if (jb.a.kind == INV_UV) {
     if (jb.var)
       inv_body<T, MODE, INV_UV, true, 1>(jb.a, blk, jb.A0, jb.A1, jb.plane, jb.tv, jb.tw, lds);
@@ -87,9 +103,9 @@ __device__ __forceinline__ void ens_inv_item(const EnsInvJob& jb, int blk, char*
   }
 }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_ens_inv(const EnsInvJob* jobs_, const int* start_,
-                                                 int njob, int* counter) {
+template <int MODE, int NT>
+__global__ __launch_bounds__(NT) void k_ens_inv(const EnsInvJob* jobs_, const int* start_,
+                                                int njob, int* counter) {
   extern __shared__ __align__(16) char lds[];
   __shared__ __align__(16) int s_item[4];  // keeps the dynamic LDS 16-byte aligned
   const ConstPtr<EnsInvJob> jobs = (ConstPtr<EnsInvJob>)jobs_;
@@ -102,26 +118,31 @@ __global__ __launch_bounds__(256) void k_ens_inv(const EnsInvJob* jobs_, const i
     const EnsInvJob& jb = *(const EnsInvJob*)(jobs + j);
     const int blk = item - start[j];
     if (jb.ntru == 63)
-      ens_inv_item<63, MODE>(jb, blk, lds);
+      ens_inv_item<63, MODE, NT>(jb, blk, lds);
     else if (jb.ntru == 42)
-      ens_inv_item<42, MODE>(jb, blk, lds);
+      ens_inv_item<42, MODE, NT>(jb, blk, lds);
     else
-      ens_inv_item<21, MODE>(jb, blk, lds);
+      ens_inv_item<21, MODE, NT>(jb, blk, lds);
     __syncthreads();  // the next item reuses the LDS
   }
 }
 
-template <int T, int MODE>
+template <int T, int MODE, int NT>
 __device__ __forceinline__ void ens_fwd_item(const EnsFwdJob& jb, int blk, char* lds) {
-  if (jb.var)
+  if (NT == 512) {
+    if (jb.var == 3)
+      fwd_body<T, MODE, true, 3>(jb.a, blk, jb.tv, jb.tw, nullptr, lds);
+    else
+      fwd_body<T, MODE, true, 2>(jb.a, blk, jb.tv, jb.tw, nullptr, lds);
+  } else if (jb.var)
     fwd_body<T, MODE, true, 1>(jb.a, blk, jb.tv, jb.tw, nullptr, lds);
   else
     fwd_body<T, MODE, true, 0>(jb.a, blk, jb.tv, jb.tw, nullptr, lds);
 }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_ens_fwd(const EnsFwdJob* jobs_, const int* start_,
-                                                 int njob, int* counter) {
+template <int MODE, int NT>
+__global__ __launch_bounds__(NT) void k_ens_fwd(const EnsFwdJob* jobs_, const int* start_,
+                                                int njob, int* counter) {
   extern __shared__ __align__(16) char lds[];
   __shared__ __align__(16) int s_item[4];  // keeps the dynamic LDS 16-byte aligned
   const ConstPtr<EnsFwdJob> jobs = (ConstPtr<EnsFwdJob>)jobs_;
@@ -134,11 +155,11 @@ __global__ __launch_bounds__(256) void k_ens_fwd(const EnsFwdJob* jobs_, const i
     const EnsFwdJob& jb = *(const EnsFwdJob*)(jobs + j);
     const int blk = item - start[j];
     if (jb.ntru == 63)
-      ens_fwd_item<63, MODE>(jb, blk, lds);
+      ens_fwd_item<63, MODE, NT>(jb, blk, lds);
     else if (jb.ntru == 42)
-      ens_fwd_item<42, MODE>(jb, blk, lds);
+      ens_fwd_item<42, MODE, NT>(jb, blk, lds);
     else
-      ens_fwd_item<21, MODE>(jb, blk, lds);
+      ens_fwd_item<21, MODE, NT>(jb, blk, lds);
     __syncthreads();
   }
 }
@@ -158,12 +179,22 @@ static bool ens_sizes(int ntru, int var, const InvArgs* ia, const FwdArgs* fa, l
 #define EXO_ENS_CASE(TT)                                                           \
   if (ntru == TT) {                                                                \
     if (ia) {                                                                      \
-      *items = var ? inv_items<TT, MODE, 1>(*ia) : inv_items<TT, MODE, 0>(*ia);    \
-      *lds = var ? InvLds<TT, MODE, true, 1>::BYTES : InvLds<TT, MODE, true, 0>::BYTES; \
+      *items = var == 3 ? inv_items<TT, MODE, 3>(*ia)                              \
+               : var == 2 ? inv_items<TT, MODE, 2>(*ia)                            \
+               : var  ? inv_items<TT, MODE, 1>(*ia) : inv_items<TT, MODE, 0>(*ia); \
+      *lds = var == 3 ? InvLds<TT, MODE, true, 3>::BYTES                           \
+             : var == 2 ? InvLds<TT, MODE, true, 2>::BYTES                         \
+             : var  ? InvLds<TT, MODE, true, 1>::BYTES : InvLds<TT, MODE, true, 0>::BYTES; \
     } else {                                                                       \
-      if (FwdCfg<TT, MODE, 0>::NPB != 1 || FwdCfg<TT, MODE, 1>::NPB != 1) return false; \
-      *items = var ? fwd_items<TT, MODE, 1>(*fa) : fwd_items<TT, MODE, 0>(*fa);    \
-      *lds = var ? FwdLds<TT, MODE, true, 1>::BYTES : FwdLds<TT, MODE, true, 0>::BYTES; \
+      if (FwdCfg<TT, MODE, 0>::NPB != 1 || FwdCfg<TT, MODE, 1>::NPB != 1 ||        \
+          FwdCfg<TT, MODE, 2>::NPB != 1 || FwdCfg<TT, MODE, 3>::NPB != 1)          \
+        return false;                                                              \
+      *items = var == 3 ? fwd_items<TT, MODE, 3>(*fa)                              \
+               : var == 2 ? fwd_items<TT, MODE, 2>(*fa)                            \
+               : var  ? fwd_items<TT, MODE, 1>(*fa) : fwd_items<TT, MODE, 0>(*fa); \
+      *lds = var == 3 ? FwdLds<TT, MODE, true, 3>::BYTES                           \
+             : var == 2 ? FwdLds<TT, MODE, true, 2>::BYTES                         \
+             : var  ? FwdLds<TT, MODE, true, 1>::BYTES : FwdLds<TT, MODE, true, 0>::BYTES; \
     }                                                                              \
     return true;                                                                   \
   }
@@ -184,13 +215,18 @@ static bool ens_sizes_mode(int mode, int ntru, int var, const InvArgs* ia, const
   }
 }
 
-static const void* ens_kernel(int mode, bool inv) {
+template <int MODE>
+static const void* ens_kernel_t(bool inv, bool wide) {
+  if (inv)
+    return wide ? (const void*)k_ens_inv<MODE, 512> : (const void*)k_ens_inv<MODE, 256>;
+  return wide ? (const void*)k_ens_fwd<MODE, 512> : (const void*)k_ens_fwd<MODE, 256>;
+}
+
+static const void* ens_kernel(int mode, bool inv, bool wide) {
   switch (mode) {
-    case MODE_F32: return inv ? (const void*)k_ens_inv<MODE_F32> : (const void*)k_ens_fwd<MODE_F32>;
-    case MODE_F32MFMA:
-      return inv ? (const void*)k_ens_inv<MODE_F32MFMA> : (const void*)k_ens_fwd<MODE_F32MFMA>;
-    case MODE_BF16X3:
-      return inv ? (const void*)k_ens_inv<MODE_BF16X3> : (const void*)k_ens_fwd<MODE_BF16X3>;
+    case MODE_F32: return ens_kernel_t<MODE_F32>(inv, wide);
+    case MODE_F32MFMA: return ens_kernel_t<MODE_F32MFMA>(inv, wide);
+    case MODE_BF16X3: return ens_kernel_t<MODE_BF16X3>(inv, wide);
     default: return nullptr;
   }
 }
@@ -219,17 +255,28 @@ struct exo_sht_ens_s {
   hipGraphExec_t exec = nullptr;
 };
 
+// the per-truncation kernels' variant closest to an ensemble variant:
+// variant 3 (half the fields, 512 threads) exists only in the persistent
+// kernels, the others run 512-thread blocks as variant 2
+static int plan_var(int v) { return v == 3 ? 2 : v; }
+
+// whether the inverse (or direct) kernel runs 512-thread blocks (tile
+// variants 2 and 3) for every group
+static bool ens_wide(const exo_sht_ens_s* e, bool inv) {
+  return e->ngroup > 0 && (inv ? e->inv_var[0] : e->fwd_var[0]) >= 2;
+}
+
 #define ENS_HIPCK(x)                                 \
   do {                                               \
     if ((x) != hipSuccess) return EXO_ERR_HIP;       \
   } while (0)
 
-static int ens_grid(const void* kern, size_t lds, int total, int* grid) {
+static int ens_grid(const void* kern, int nt, size_t lds, int total, int* grid) {
   int dev = 0, ncu = 0, nb = 0;
   ENS_HIPCK(hipGetDevice(&dev));
   ENS_HIPCK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   ENS_HIPCK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  ENS_HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, lds));
+  ENS_HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, nt, lds));
   if (nb < 1) return EXO_ERR_UNSUPPORTED;
   *grid = std::max(1, std::min(total, nb * ncu));
   return EXO_OK;
@@ -362,20 +409,33 @@ static int ens_prepare(exo_sht_ens_s* e) {
   e->lds_fwd = lds_fwd;
   int rc = EXO_OK;
   if (ii.start[ii.njob] > 0)
-    rc = ens_grid(ens_kernel(e->mode, true), lds_inv, ii.start[ii.njob], &e->grid_inv);
+    rc = ens_grid(ens_kernel(e->mode, true, ens_wide(e, true)), ens_wide(e, true) ? 512 : 256,
+                  lds_inv, ii.start[ii.njob], &e->grid_inv);
   if (rc == EXO_OK && fi.start[fi.njob] > 0)
-    rc = ens_grid(ens_kernel(e->mode, false), lds_fwd, fi.start[fi.njob], &e->grid_fwd);
+    rc = ens_grid(ens_kernel(e->mode, false, ens_wide(e, false)), ens_wide(e, false) ? 512 : 256,
+                  lds_fwd, fi.start[fi.njob], &e->grid_fwd);
   return rc;
 }
 
 template <int MODE>
 static void ens_launch_t(const exo_sht_ens_s* e, bool inv, hipStream_t s) {
-  if (inv)
-    k_ens_inv<MODE><<<e->grid_inv, 256, e->lds_inv, s>>>(e->dinv, e->dstart, e->iinv.njob,
-                                                         e->counter);
-  else
-    k_ens_fwd<MODE><<<e->grid_fwd, 256, e->lds_fwd, s>>>(e->dfwd, e->dstart + ENS_MAXJOB + 1,
-                                                         e->ifwd.njob, e->counter + 1);
+  const bool wide = ens_wide(e, inv);
+  if (inv) {
+    if (wide)
+      k_ens_inv<MODE, 512><<<e->grid_inv, 512, e->lds_inv, s>>>(e->dinv, e->dstart, e->iinv.njob,
+                                                                e->counter);
+    else
+      k_ens_inv<MODE, 256><<<e->grid_inv, 256, e->lds_inv, s>>>(e->dinv, e->dstart, e->iinv.njob,
+                                                                e->counter);
+  } else {
+    const int* st = e->dstart + ENS_MAXJOB + 1;
+    if (wide)
+      k_ens_fwd<MODE, 512><<<e->grid_fwd, 512, e->lds_fwd, s>>>(e->dfwd, st, e->ifwd.njob,
+                                                                e->counter + 1);
+    else
+      k_ens_fwd<MODE, 256><<<e->grid_fwd, 256, e->lds_fwd, s>>>(e->dfwd, st, e->ifwd.njob,
+                                                                e->counter + 1);
+  }
 }
 
 static int ens_launch(exo_sht_ens_s* e, bool inv, hipStream_t s) {
@@ -522,7 +582,8 @@ extern "C" int exo_sht_ens_run_groups(exo_sht_ens_t e, void* stream) {
   if (!e) return EXO_ERR_ARG;
   for (int k = 0; k < e->ngroup; ++k) {
     const exo_sht_ens_group& g = e->g[k];
-    int rc = exo_sht_set_tiles(e->plan[k], e->mode, e->inv_var[k], e->inv_var[k], e->fwd_var[k]);
+    int rc = exo_sht_set_tiles(e->plan[k], e->mode, plan_var(e->inv_var[k]), plan_var(e->inv_var[k]),
+                               plan_var(e->fwd_var[k]));
     if (rc == EXO_OK)
       rc = exo_sht_roundtrip(e->plan[k], e->mode, g.nmem, g.nlev, g.sp_in, g.gp, g.sp_out, stream);
     if (rc != EXO_OK) return rc;
@@ -538,7 +599,8 @@ extern "C" int exo_sht_ens_run_members(exo_sht_ens_t e, void* stream) {
     int dims[4];
     int rc = exo_sht_plan_dims(e->plan[k], dims);
     if (rc == EXO_OK)
-      rc = exo_sht_set_tiles(e->plan[k], e->mode, e->inv_var[k], e->inv_var[k], e->fwd_var[k]);
+      rc = exo_sht_set_tiles(e->plan[k], e->mode, plan_var(e->inv_var[k]), plan_var(e->inv_var[k]),
+                               plan_var(e->fwd_var[k]));
     if (rc != EXO_OK) return rc;
     const int nfl = 3 * g.nlev + 1, ngp = nfl + 1 + 2 * g.nlev;
     const long sps = (long)nfl * 2 * dims[3], gps = (long)ngp * dims[1] * dims[2];
@@ -565,7 +627,14 @@ extern "C" int exo_sht_ens_get_tiles(exo_sht_ens_t e, int* vars) {
 extern "C" int exo_sht_ens_set_tiles(exo_sht_ens_t e, const int* vars) {
   if (!e || !vars) return EXO_ERR_ARG;
   for (int k = 0; k < 2 * e->ngroup; ++k)
-    if (vars[k] < 0 || vars[k] > 1) return EXO_ERR_ARG;
+    if (vars[k] < 0 || vars[k] > 3) return EXO_ERR_ARG;
+  // variants 2 and 3 (512-thread blocks) hold for all groups of a kernel or
+  // for none
+  for (int w = 0; w < 2; ++w) {
+    int n2 = 0;
+    for (int k = 0; k < e->ngroup; ++k) n2 += vars[2 * k + w] >= 2;
+    if (n2 != 0 && n2 != e->ngroup) return EXO_ERR_ARG;
+  }
   (void)hipDeviceSynchronize();
   ens_drop_graph(e);  // stale once the job tables change
   for (int k = 0; k < e->ngroup; ++k) {
@@ -578,8 +647,9 @@ extern "C" int exo_sht_ens_set_tiles(exo_sht_ens_t e, const int* vars) {
 }
 
 // Times each combination of tile variants per truncation (the groups of one
-// truncation share a variant) for the inverse and the direct kernel, keeps
-// the fastest and captures the graph again.
+// truncation share a variant: 0 or 1 with 256-thread blocks, 2 or 3 with
+// 512-thread blocks) for the inverse and the direct kernel, keeps the
+// fastest and captures the graph again.
 extern "C" int exo_sht_ens_autotune(exo_sht_ens_t e, void* stream, int* vars) {
   if (!e) return EXO_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
@@ -594,13 +664,15 @@ extern "C" int exo_sht_ens_autotune(exo_sht_ens_t e, void* stream, int* vars) {
   int pick[2] = {0, 0};
   int rc0 = ens_pack(e, s);  // the inverse timings read the pre-split inputs
   if (rc0 != EXO_OK) rc = rc0;
-  for (int combo = 0; combo < 8 && rc == EXO_OK; ++combo) {
+  // combinations 0-7: variant 0 or 1 per truncation (256-thread blocks);
+  // 8-15: variant 2 or 3 per truncation (512-thread blocks)
+  for (int combo = 0; combo < 16 && rc == EXO_OK; ++combo) {
     (void)hipStreamSynchronize(s);
     for (int k = 0; k < e->ngroup; ++k) {
       int bit = 0;
       for (int t = 0; t < 3; ++t)
         if (e->g[k].ntru == truncs[t]) bit = (combo >> t) & 1;
-      e->inv_var[k] = e->fwd_var[k] = bit;
+      e->inv_var[k] = e->fwd_var[k] = (combo >> 3) * 2 + bit;
     }
     rc = ens_prepare(e);
     for (int which = 0; which < 2 && rc == EXO_OK; ++which) {
@@ -630,8 +702,8 @@ extern "C" int exo_sht_ens_autotune(exo_sht_ens_t e, void* stream, int* vars) {
         bi = (pick[0] >> t) & 1;
         bf = (pick[1] >> t) & 1;
       }
-    e->inv_var[k] = bi;
-    e->fwd_var[k] = bf;
+    e->inv_var[k] = (pick[0] >> 3) * 2 + bi;
+    e->fwd_var[k] = (pick[1] >> 3) * 2 + bf;
   }
   if (rc == EXO_OK) rc = ens_prepare(e);
   if (rc == EXO_OK) rc = ens_capture(e);

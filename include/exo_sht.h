@@ -262,10 +262,12 @@ int exo_sht_ens_run_groups(exo_sht_ens_t ens, void *stream);
 /* the same round trips one member at a time (4 launches per member) */
 int exo_sht_ens_run_members(exo_sht_ens_t ens, void *stream);
 /* tile variants per group, vars[2*k] inverse and vars[2*k+1] direct of
- * group k (0 or 1 as for exo_sht_set_tiles; the persistent kernels run
- * 256-thread blocks); set_tiles captures the graph
- * again.  autotune times every combination per truncation, keeps the
- * fastest and returns it in vars (may be NULL). */
+ * group k: 0, 1 or 2 as for exo_sht_set_tiles, and 3 = half the
+ * field-levels with 512-thread blocks.  Variants 2 and 3 run the persistent
+ * kernel with 512-thread blocks, so they hold for every group of that
+ * kernel or for none (else EXO_ERR_ARG).  set_tiles captures the graph
+ * again.  autotune times every combination per truncation (0/1, and 2/3),
+ * keeps the fastest and returns it in vars (may be NULL). */
 int exo_sht_ens_get_tiles(exo_sht_ens_t ens, int *vars);
 int exo_sht_ens_set_tiles(exo_sht_ens_t ens, const int *vars);
 int exo_sht_ens_autotune(exo_sht_ens_t ens, void *stream, int *vars);

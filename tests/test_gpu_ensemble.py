@@ -103,8 +103,28 @@ def test_ensemble_tiles_and_autotune(oracle, gpu):
         for k in range(len(GROUPS)):
             assert rel_l2(half[k][0], base[k][0]) < 1e-6
             assert rel_l2(half[k][1], base[k][1]) < 1e-6
+        # variant 2: the persistent kernels with 512-thread blocks; it holds
+        # for every group of a kernel or for none
+        ens.set_tiles([2] * (2 * len(GROUPS)))
+        _poison(ens)
+        ens.run()
+        wide = _outputs(ens)
+        for k in range(len(GROUPS)):
+            assert rel_l2(wide[k][0], base[k][0]) < 1e-6
+            assert rel_l2(wide[k][1], base[k][1]) < 1e-6
+        ens.set_tiles([3] * (2 * len(GROUPS)))
+        _poison(ens)
+        ens.run()
+        wide3 = _outputs(ens)
+        for k in range(len(GROUPS)):
+            assert rel_l2(wide3[k][0], base[k][0]) < 1e-6
+            assert rel_l2(wide3[k][1], base[k][1]) < 1e-6
+        with pytest.raises(sht.ExoError):
+            ens.set_tiles([2, 0] + [0] * (2 * len(GROUPS) - 2))
         v = ens.autotune()
-        assert all(x in (0, 1) for x in v) and ens.tiles() == v
+        assert all(x in (0, 1, 2, 3) for x in v) and ens.tiles() == v
+        for w in (0, 1):  # 512-thread variants for all groups of a kernel or for none
+            assert len({x >= 2 for x in v[w::2]}) == 1
         # groups of one truncation share a variant
         _poison(ens)
         ens.run()
