@@ -29,6 +29,11 @@ struct InvCfg {
       (T == 127) ? (MODE == MODE_F64 ? 2 : 4) : (MODE == MODE_F64 ? 4 : 8);
   static constexpr int FB = (V == 1 || V == 3) ? ((FB0 >> 1) < 2 ? 2 : (FB0 >> 1)) : FB0;
   static constexpr int NT = (V == 2 || V == 3) ? 512 : 256;  // threads per block
+  // MFMA work items in flight per wave, one table (D1) or two (D2; dv2uv
+  // carries two tables per item, and one item ahead keeps its registers
+  // within four waves per SIMD)
+  static constexpr int D1 = 2;
+  static constexpr int D2 = 1;
 };
 
 template <int T, int MODE, int V = 0>
@@ -395,45 +400,25 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
         acc1 = acc1b = acc2 = acc2b = zero4;
       }
     };
-    if constexpr (NTERM == 1) {
-      // three operand buffers in rotation, unrolled so that no buffer is
-      // copied: two items are in flight while one is multiplied
-      Frag f0, f1, f2;
-      Item i0 = {wv, 0};
-      Item i1 = next_item(i0), i2 = next_item(i1);
-      fetch(i0.m, i0.kb, f0.b, f0.a0, f0.a1);
-      fetch(i1.m, i1.kb, f1.b, f1.a0, f1.a1);
-      while (i0.m <= T) {
-        step(i0, f0, f2, i2);
-        i0 = i1;
-        i1 = i2;
-        i2 = next_item(i2);
-        if (i0.m > T) break;
-        step(i0, f1, f0, i2);
-        i0 = i1;
-        i1 = i2;
-        i2 = next_item(i2);
-        if (i0.m > T) break;
-        step(i0, f2, f1, i2);
-        i0 = i1;
-        i1 = i2;
-        i2 = next_item(i2);
-      }
-    } else {
-      // dv2uv carries two tables per item: two buffers (one item ahead)
-      // keep the registers within four waves per SIMD
-      Frag f0, f1;
-      Item i0 = {wv, 0};
-      Item i1 = next_item(i0);
-      fetch(i0.m, i0.kb, f0.b, f0.a0, f0.a1);
-      while (i0.m <= T) {
-        step(i0, f0, f1, i1);
-        i0 = i1;
-        i1 = next_item(i1);
-        if (i0.m > T) break;
-        step(i0, f1, f0, i1);
-        i0 = i1;
-        i1 = next_item(i1);
+    // DEPTH + 1 operand buffers in rotation, unrolled so that no buffer is
+    // copied: DEPTH items are in flight while one is multiplied.  The step
+    // of round r multiplies buffer r and fetches DEPTH items ahead into
+    // buffer (r + DEPTH) mod (DEPTH + 1).
+    constexpr int DEPTH = (NTERM == 1) ? InvCfg<T, MODE, V>::D1 : InvCfg<T, MODE, V>::D2;
+    Frag fr[DEPTH + 1];
+    Item cur = {wv, 0}, ahead = cur;
+#pragma unroll
+    for (int k = 0; k < DEPTH; ++k) {
+      fetch(ahead.m, ahead.kb, fr[k].b, fr[k].a0, fr[k].a1);
+      ahead = next_item(ahead);
+    }
+    while (cur.m <= T) {
+#pragma unroll
+      for (int r = 0; r <= DEPTH; ++r) {
+        if (r > 0 && cur.m > T) break;
+        step(cur, fr[r], fr[(r + DEPTH) % (DEPTH + 1)], ahead);
+        cur = next_item(cur);
+        ahead = next_item(ahead);
       }
     }
   } else {
