@@ -435,7 +435,7 @@ extern "C" int exo_gridpointa(exo_sht_plan_t p, exo_dyn_t d, int mode, int nmem,
   if (mode == MODE_BF16X3) {
     const int rc = plan_grow(&p->pk, &p->pk_bytes, pack_bytes(tb, nmem, nsp));
     if (rc) return rc;
-    if (launch_pack(tb, (const float*)sp_state, sp_stride, 0, nsp, nmem, (uint16_t*)p->pk, s))
+    if (launch_pack(tb, (const float*)sp_state, sp_stride, 0, nsp, nmem, (float*)p->pk, s))
       return EXO_ERR_LAUNCH;
     base.bpk = p->pk;
     base.bpk_stride = (long)nsp * pk_field(tb.ntru);

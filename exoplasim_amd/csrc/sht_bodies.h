@@ -222,21 +222,24 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     const bool amine0 = hq == ((c & 1) ^ (table_is_q(KIND == INV_UV ? TQV : a.table) ? 1 : 0));
     const bool amine1 = hq == ((c & 1) ^ (table_is_q(TQU) ? 1 : 0));
     const int avoff = (q * 16 + c) * 32;  // F32MFMA
-    // B (BF16X3): the member's pre-split operand; columns without a field
-    // read past the buffer, which returns zeros
+    // B (BF16X3): the member's parity-ordered fp32 operand; columns without
+    // a field read past the buffer, which returns zeros
     __amdgpu_buffer_rsrc_t rsB = rsA0;
     int bvoff = BUF_NONE;
     if constexpr (MODE == MODE_BF16X3) {
-      rsB = buffer_rsrc((const uint16_t*)a.bpk + mem * a.bpk_stride, (int)a.bpk_stride * 2);
-      if (pf >= 0) bvoff = ((pf - a.bpk_f0) * pk_field(T) + jq) * 2;
+      rsB = buffer_rsrc((const float*)a.bpk + mem * a.bpk_stride, (int)a.bpk_stride * 4);
+      if (pf >= 0) bvoff = ((pf - a.bpk_f0) * pk_field(T) + jq) * 4;
     }
     const int brow = (c & 1) * 2 + hq;  // re/im, parity row of the B segment
     const R pv = (R)a.plavor;
-    // B fragment of one item: 8 fp32 values (F32MFMA, n = 4j + q) or the
-    // three bf16 planes of 8 values of one parity (BF16X3)
+    // B fragment of one item: 8 fp32 values (F32MFMA, n = 4j + q), or 8
+    // fp32 values of one parity as loaded (BF16X3: r, split into the three
+    // bf16 planes when the item is multiplied; fix marks the lane that holds
+    // the absolute vorticity at (m=0, n=1, re))
     struct BFrag {
       float f[8];
-      BF8 p[3];
+      uint4 r[2];
+      bool fix;
     };
     // Every step fetches, also past the last item (m > T: nothing is loaded,
     // at the offsets of m = T), so that each step issues the same loads and
@@ -250,19 +253,13 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
         // lanes whose j lies past the row would read n > NTRU, where the
         // table operand is zero: they load nothing
         const bool inrow = live && 16 * kb + jq < rw;
-        const int bso = (pk_seg(T, m) + 16 * kb) * 2;
-        const int bv = inrow ? bvoff + brow * rw * 2 : BUF_NONE;
-#pragma unroll
-        for (int p = 0; p < 3; ++p) br.p[p].q = buffer_load_x4(rsB, bv, bso + p * rw * 8);
-        if (m == 0 && kb == 0 && isvor && q == 2) {
-          // absolute -> relative vorticity at (m=0, n=1, re), the first odd
-          // value: the split is exact, so the sum of the planes is the fp32
-          // coefficient
-          const float x = __uint_as_float((uint32_t)br.p[0].s[0] << 16) +
-                          __uint_as_float((uint32_t)br.p[1].s[0] << 16) +
-                          __uint_as_float((uint32_t)br.p[2].s[0] << 16);
-          split3(x - (float)pv, br.p[0].s[0], br.p[1].s[0], br.p[2].s[0]);
-        }
+        const int bso = (pk_seg(T, m) + 16 * kb) * 4;
+        const int bv = inrow ? bvoff + brow * rw * 4 : BUF_NONE;
+        br.r[0] = buffer_load_x4(rsB, bv, bso);
+        br.r[1] = buffer_load_x4(rsB, bv, bso + 16);
+        // absolute -> relative vorticity at (m=0, n=1, re), the first odd
+        // value (subtracted when the item is multiplied)
+        br.fix = live && m == 0 && kb == 0 && isvor && q == 2;
         const int aso = (tab_seg(T, m) + 16 * kb) * 2;
         const int av = (c * rw + jq) * 2;
         const int av0 = (amine0 && inrow) ? av : BUF_NONE;
@@ -298,21 +295,33 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
         }
       }
     };
+    // BF16X3: the exact three-way bf16 split of an item's 8 B values
+    auto split = [&](const BFrag& br, BF8* bp) {
+      if constexpr (MODE == MODE_BF16X3) {
+        float x[8] = {__uint_as_float(br.r[0].x), __uint_as_float(br.r[0].y),
+                      __uint_as_float(br.r[0].z), __uint_as_float(br.r[0].w),
+                      __uint_as_float(br.r[1].x), __uint_as_float(br.r[1].y),
+                      __uint_as_float(br.r[1].z), __uint_as_float(br.r[1].w)};
+        if (br.fix) x[0] = x[0] - (float)pv;
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) split3(x[jj], bp[0].s[jj], bp[1].s[jj], bp[2].s[jj]);
+      }
+    };
     // two independent accumulation chains per term (the small cross terms
     // and the large ones), so that consecutive MFMAs do not all wait for the
     // previous result
-    auto mul = [&](const BFrag& br, const uint4* a, f32x4& acc, f32x4& acc2) {
+    auto mul = [&](const BFrag& br, const BF8* bp, const uint4* a, f32x4& acc, f32x4& acc2) {
       if constexpr (MODE == MODE_BF16X3) {
         BF8 ah, am, al;
         ah.q = a[0];
         am.q = a[1];
         al.q = a[2];
-        acc = mfma_bf16(ah, br.p[2], acc);
-        acc2 = mfma_bf16(am, br.p[0], acc2);
-        acc = mfma_bf16(am, br.p[1], acc);
-        acc2 = mfma_bf16(ah, br.p[1], acc2);
-        acc = mfma_bf16(al, br.p[0], acc);
-        acc2 = mfma_bf16(ah, br.p[0], acc2);
+        acc = mfma_bf16(ah, bp[2], acc);
+        acc2 = mfma_bf16(am, bp[0], acc2);
+        acc = mfma_bf16(am, bp[1], acc);
+        acc2 = mfma_bf16(ah, bp[1], acc2);
+        acc = mfma_bf16(al, bp[0], acc);
+        acc2 = mfma_bf16(ah, bp[0], acc2);
       } else {
         const float* af = (const float*)a;
 #pragma unroll
@@ -341,8 +350,10 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     // m, combine the terms, fold N/S and store the Fourier coefficients
     auto step = [&](Item cur, Frag& use, Frag& into, Item ahead) {
       fetch(ahead.m, ahead.kb, into.b, into.a0, into.a1);
-      mul(use.b, use.a0, acc1, acc1b);
-      if (NTERM == 2) mul(use.b, use.a1, acc2, acc2b);
+      BF8 bp[3];
+      split(use.b, bp);
+      mul(use.b, bp, use.a0, acc1, acc1b);
+      if (NTERM == 2) mul(use.b, bp, use.a1, acc2, acc2b);
       if (next_item(cur).kb == 0) {
         const int m = cur.m;
         acc1 += acc1b;

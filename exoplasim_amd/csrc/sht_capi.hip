@@ -263,7 +263,7 @@ static int pack_for(exo_sht_plan_t p, int mode, int nmem, const void* sp, long s
     if (rc != EXO_OK) return rc;
     buf = p->pk;
   }
-  if (launch_pack(p->tb, (const float*)sp, sps, f0, nf, nmem, (uint16_t*)buf, s) != 0)
+  if (launch_pack(p->tb, (const float*)sp, sps, f0, nf, nmem, (float*)buf, s) != 0)
     return EXO_ERR_LAUNCH;
   a.bpk = buf;
   a.bpk_stride = (long)nf * pk_field(p->tb.ntru);

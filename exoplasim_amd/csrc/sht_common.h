@@ -74,10 +74,11 @@ __host__ __device__ constexpr int pk_c16(int x) {
   const unsigned a = (unsigned)x >> 4, b = (unsigned)x & 15u;
   return (int)(64u * a * (a + 1u) + 8u * b * (a + 1u));
 }
-// spectral operand: per field, for each m, [plane hi/mid/lo][re/im][h][pk_rw]
-__host__ __device__ constexpr int pk_field(int ntru) { return 12 * pk_c16(ntru + 1); }
+// spectral operand: per field, for each m, [re/im][h][pk_rw] fp32 values in
+// parity order (the inverse splits them into bf16 planes in registers)
+__host__ __device__ constexpr int pk_field(int ntru) { return 4 * pk_c16(ntru + 1); }
 __host__ __device__ constexpr int pk_seg(int ntru, int m) {
-  return 12 * (pk_c16(ntru + 1) - pk_c16(ntru + 1 - m));
+  return 4 * (pk_c16(ntru + 1) - pk_c16(ntru + 1 - m));
 }
 // table operand: per block of 8 latitude pairs, for each m,
 // [plane][row = 2 pair + class][pk_rw]; the row holds the parity of its

@@ -240,7 +240,7 @@ struct exo_sht_ens_s {
   int ngroup = 0;
   exo_sht_ens_group g[ENS_MAXGROUP];
   exo_sht_plan_t plan[ENS_MAXGROUP] = {};  // one per group (own tables)
-  uint16_t* pk[ENS_MAXGROUP] = {};         // BF16X3: pre-split spectral input per group
+  float* pk[ENS_MAXGROUP] = {};            // BF16X3: parity-ordered spectral input per group
   int inv_var[ENS_MAXGROUP] = {}, fwd_var[ENS_MAXGROUP] = {};
   // device state
   EnsInvJob* dinv = nullptr;

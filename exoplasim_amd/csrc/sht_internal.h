@@ -59,8 +59,8 @@ struct InvArgs {
   double plavor;
   int var;          // tile variant (0 default, 1 half the fields per block)
   int dlam;         // INV_SCALAR: times i m before the FFT (d/dlambda)
-  // MODE_BF16X3: the pre-split spectral operand (launch_pack) of fields
-  // [bpk_f0, ...), bpk_stride bf16 values per member
+  // MODE_BF16X3: the parity-ordered spectral operand (launch_pack) of
+  // fields [bpk_f0, ...), bpk_stride floats per member
   const void* bpk;
   long bpk_stride;
   int bpk_f0;
@@ -130,13 +130,13 @@ size_t fwd_fft_scratch_bytes(const Tables& T, int nmem, int nfield);
 int launch_fft(const Tables& T, int mode, int dir, void* a, int n, long lot, hipStream_t s,
                int nmem = 1, long mstride = 0, const void* src = nullptr, long smstride = 0,
                int kmax = 0);
-// pre-split copy (PK layout, sht_common.h) of fp32 spectral fields [f0, f0+nf)
-// of nmem members for the split-BF16 inverse; out holds nmem * nf *
-// pk_field(ntru) bf16 values
+// parity-ordered copy (PK layout, sht_common.h) of fp32 spectral fields
+// [f0, f0+nf) of nmem members for the split-BF16 inverse; out holds nmem *
+// nf * pk_field(ntru) floats
 int launch_pack(const Tables& T, const float* sp, long sp_stride, int f0, int nf, int nmem,
-                uint16_t* out, hipStream_t s);
+                float* out, hipStream_t s);
 inline size_t pack_bytes(const Tables& T, int nmem, int nf) {
-  return sizeof(uint16_t) * (size_t)nmem * nf * pk_field(T.ntru);
+  return sizeof(float) * (size_t)nmem * nf * pk_field(T.ntru);
 }
 
 // the device tables of a plan (defined with the plan object)

@@ -201,14 +201,14 @@ static int fft_t(const void* tw, int dir, R* a, long lot, int nmem, long mstride
 }
 
 // ---------------------------------------------------------------------------
-// pre-split spectral operand of the split-BF16 inverse (layout in
+// parity-ordered spectral operand of the split-BF16 inverse (layout in
 // sht_common.h): one thread per 8 consecutive values of one (member, field,
-// m, re/im, parity) row; each of the three planes is written as one 16-byte
-// chunk, so consecutive threads fill a plane row contiguously
+// m, re/im, parity) row, written as two 16-byte chunks, so consecutive
+// threads fill a row contiguously
 // ---------------------------------------------------------------------------
 template <int T>
 __global__ __launch_bounds__(256) void k_pack_sp(const float* sp, long sp_stride, int f0, int nf,
-                                                 int nmem, uint16_t* out) {
+                                                 int nmem, float* out) {
   using S = TS<T>;
   constexpr int NC = pk_c16(T + 1) / 2;  // 8-value chunks per field (4 rows per m)
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
@@ -220,31 +220,29 @@ __global__ __launch_bounds__(256) void k_pack_sp(const float* sp, long sp_stride
   int lo = 0, hi = T;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (pk_seg(T, mid) / 24 <= u)
+    if (pk_seg(T, mid) / 8 <= u)
       lo = mid;
     else
       hi = mid - 1;
   }
   const int m = lo, rw = pk_rw(T, m);
-  const int r = u - pk_seg(T, m) / 24;  // chunk within the segment's 4 rows
+  const int r = u - pk_seg(T, m) / 8;  // chunk within the segment's 4 rows
   const int row = r / (rw / 8), j0 = (r - row * (rw / 8)) * 8;
   const int ri = row >> 1, h = row & 1;
   const float* src = sp + mem * sp_stride + (long)(f0 + field) * S::NRSP + 2 * w0_of(T, m) + ri;
-  BF8 hi8, md8, lo8;
+  float x[8];
 #pragma unroll
   for (int jj = 0; jj < 8; ++jj) {
     const int dn = 2 * (j0 + jj) + h;  // n - m
-    const float x = (m + dn <= T) ? src[2 * dn] : 0.f;
-    split3(x, hi8.s[jj], md8.s[jj], lo8.s[jj]);
+    x[jj] = (m + dn <= T) ? src[2 * dn] : 0.f;
   }
-  uint16_t* dst = out + fm * pk_field(T) + pk_seg(T, m) + row * rw + j0;
-  *(uint4*)dst = hi8.q;
-  *(uint4*)(dst + 4 * rw) = md8.q;
-  *(uint4*)(dst + 8 * rw) = lo8.q;
+  float* dst = out + fm * pk_field(T) + pk_seg(T, m) + row * rw + j0;
+  *(float4*)dst = make_float4(x[0], x[1], x[2], x[3]);
+  *(float4*)(dst + 4) = make_float4(x[4], x[5], x[6], x[7]);
 }
 
 template <int T>
-static int pack_t(const float* sp, long sps, int f0, int nf, int nmem, uint16_t* out,
+static int pack_t(const float* sp, long sps, int f0, int nf, int nmem, float* out,
                   hipStream_t s) {
   const long total = (long)nmem * nf * (pk_c16(T + 1) / 2);
   if (total <= 0) return 0;
@@ -253,7 +251,7 @@ static int pack_t(const float* sp, long sps, int f0, int nf, int nmem, uint16_t*
 }
 
 int launch_pack(const Tables& tb, const float* sp, long sp_stride, int f0, int nf, int nmem,
-                uint16_t* out, hipStream_t s) {
+                float* out, hipStream_t s) {
   switch (tb.ntru) {
     case 21: return pack_t<21>(sp, sp_stride, f0, nf, nmem, out, s);
     case 42: return pack_t<42>(sp, sp_stride, f0, nf, nmem, out, s);

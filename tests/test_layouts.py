@@ -69,7 +69,7 @@ static void check_layout(int T) {
     for (int L = 1; L <= x; ++L) s += 8 * ((L + 15) / 16);
     CHECK(pk_c16(x) == s);
   }
-  // segments: spectral [plane 3][re/im 2][parity 2][rw], table [plane 3][row 16][rw]
+  // segments: spectral [re/im 2][parity 2][rw] fp32, table [plane 3][row 16][rw]
   int pk = 0, tab = 0;
   for (int m = 0; m <= T; ++m) {
     const int rw = pk_rw(T, m);
@@ -78,7 +78,7 @@ static void check_layout(int T) {
     CHECK(rw % 8 == 0 && 2 * rw >= L);   // a parity row holds ceil(L/2) values
     CHECK(pk_seg(T, m) == pk);
     CHECK(tab_seg(T, m) == tab);
-    pk += 12 * rw;
+    pk += 4 * rw;
     tab += 48 * rw;
     // every n - m < L lies in K block kb = (n - m) / 32, at j = (n - m) / 2
     // within [16 kb, 16 kb + 16) and below rw
