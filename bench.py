@@ -246,9 +246,9 @@ def bench_t127(args, ora):
     es = sp_in.element_size()
     nbytes = nmem * (nfl + 1 + 2 * nlev + nfl) * (plan.nrsp + plan.nlat * plan.nlon) * es
     t_qi = time_loop(lambda: plan.inverse(mode, sp_in, gp, nfl, 0, 0, "qi"), steps, 2) / steps
-    # fields per block of the qi launch: 4 at T127, halved by tile variant 1
-    # only (variant 2 keeps the fields and doubles the threads)
-    fb = 2 if tiles[0] == 1 else 4
+    # fields per block of the qi launch: 4 at T127, halved by tile variants 1
+    # and 3 (variant 2 keeps the fields and doubles the threads)
+    fb = 2 if tiles[0] in (1, 3) else 4
     fl_qi = _mfma_flops_inverse(ntru, nfl, nmem, fb, 1)
     torch.cuda.synchronize()
     plan.roundtrip(mode, sp_in, gp, sp_out, nlev)
@@ -556,7 +556,7 @@ def main():
     inv_mfma = None
     if args.mode == "bf16x3":
         t_inv = head["kernels"]["inverse_all"]["ms"] / 1e3
-        fb = 4 if tiles[args.mode][0] == 1 else 8  # tile variant 1 halves the fields
+        fb = 4 if tiles[args.mode][0] in (1, 3) else 8  # tile variants 1 and 3 halve the fields
         ex = (_mfma_flops_inverse(NTRU, nfl, nmem, fb, 1) + _mfma_flops_inverse(NTRU, 1, nmem, fb, 1)
               + _mfma_flops_inverse(NTRU, NLEV, nmem, fb // 2, 2))
         per = 2 * (nrsp // 2) * (plan.nlat // 2) * 2  # flops per field and component set

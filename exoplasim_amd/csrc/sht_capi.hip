@@ -472,8 +472,8 @@ extern "C" int exo_sht_roundtrip_noalloc(exo_sht_plan_t p, int mode, int nmem, i
 
 extern "C" int exo_sht_set_tiles(exo_sht_plan_t p, int mode, int inv_var, int uv_var,
                                  int fwd_var) {
-  if (!p || !mode_ok(mode) || inv_var < 0 || inv_var > 2 || uv_var < 0 || uv_var > 2 ||
-      fwd_var < 0 || fwd_var > 2)
+  if (!p || !mode_ok(mode) || inv_var < 0 || inv_var > 3 || uv_var < 0 || uv_var > 3 ||
+      fwd_var < 0 || fwd_var > 3)
     return EXO_ERR_ARG;
   std::lock_guard<std::mutex> lk(p->mu);
   p->inv_var[mode] = inv_var;
@@ -578,7 +578,7 @@ extern "C" int exo_sht_autotune(exo_sht_plan_t p, int mode, int nmem, int nlev, 
   // each variant of each kernel: 2 untimed calls, then the average of 8
   // back-to-back calls (steady state, no host round trip between them)
   constexpr int WARM = 2, REPS = 8;
-  for (int v = 0; v < 3 && rc == EXO_OK; ++v) {
+  for (int v = 0; v < 4 && rc == EXO_OK; ++v) {
     p->inv_var[mode] = p->uv_var[mode] = p->fwd_var[mode] = v;
     for (int k = 0; k < 3 && rc == EXO_OK; ++k) {
       for (int rep = 0; rep < WARM + REPS && rc == EXO_OK; ++rep) {

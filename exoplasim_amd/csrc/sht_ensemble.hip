@@ -255,10 +255,8 @@ struct exo_sht_ens_s {
   hipGraphExec_t exec = nullptr;
 };
 
-// the per-truncation kernels' variant closest to an ensemble variant:
-// variant 3 (half the fields, 512 threads) exists only in the persistent
-// kernels, the others run 512-thread blocks as variant 2
-static int plan_var(int v) { return v == 3 ? 2 : v; }
+// the per-truncation kernels run the same variants
+static int plan_var(int v) { return v; }
 
 // whether the inverse (or direct) kernel runs 512-thread blocks (tile
 // variants 2 and 3) for every group

@@ -102,6 +102,7 @@ template <int T, int MODE>
 static int inv_t(const Tables& tb, bool fused, const InvArgs& a, hipStream_t s) {
   if (fused && a.var == 1) return inv_tv<T, MODE, 1>(tb, fused, a, s);
   if (fused && a.var == 2) return inv_tv<T, MODE, 2>(tb, fused, a, s);
+  if (fused && a.var == 3) return inv_tv<T, MODE, 3>(tb, fused, a, s);
   return inv_tv<T, MODE, 0>(tb, fused, a, s);
 }
 
@@ -140,6 +141,7 @@ static int inv_rt_t(const Tables& tb, const InvArgs& aq, const InvArgs& aj, cons
                     hipStream_t s) {
   if (aq.var == 1) return inv_rt_tv<T, MODE, 1>(tb, aq, aj, au, s);
   if (aq.var == 2) return inv_rt_tv<T, MODE, 2>(tb, aq, aj, au, s);
+  if (aq.var == 3) return inv_rt_tv<T, MODE, 3>(tb, aq, aj, au, s);
   return inv_rt_tv<T, MODE, 0>(tb, aq, aj, au, s);
 }
 
@@ -179,6 +181,7 @@ static int fwd_t(const Tables& tb, bool fused, const FwdArgs& a, void* scratch, 
                  hipStream_t s) {
   if (fused && a.var == 1) return fwd_tv<T, MODE, 1>(tb, fused, a, scratch, sbytes, s);
   if (fused && a.var == 2) return fwd_tv<T, MODE, 2>(tb, fused, a, scratch, sbytes, s);
+  if (fused && a.var == 3) return fwd_tv<T, MODE, 3>(tb, fused, a, scratch, sbytes, s);
   return fwd_tv<T, MODE, 0>(tb, fused, a, scratch, sbytes, s);
 }
 

@@ -221,7 +221,8 @@ int exo_sht_mktend(exo_sht_plan_t plan, int mode, int nmem, int nlev, void *d, v
  * exo_sht_roundtrip, vars[1] the dv2uv kernel of exo_sht_inverse_uv,
  * vars[2] the direct kernel; 0 = default tiles, 1 = half the field-levels
  * per block (half the LDS, twice the blocks in flight), 2 = default tiles
- * on 512-thread blocks (twice the waves per block).  exo_sht_autotune
+ * on 512-thread blocks (twice the waves per block), 3 = half the
+ * field-levels on 512-thread blocks.  exo_sht_autotune
  * times each variant (two untimed calls, then the average of eight
  * back-to-back calls) on a round trip of nmem members x nlev levels, keeps
  * the fastest of each for this mode and returns them in vars[3] (may be
@@ -262,8 +263,7 @@ int exo_sht_ens_run_groups(exo_sht_ens_t ens, void *stream);
 /* the same round trips one member at a time (4 launches per member) */
 int exo_sht_ens_run_members(exo_sht_ens_t ens, void *stream);
 /* tile variants per group, vars[2*k] inverse and vars[2*k+1] direct of
- * group k: 0, 1 or 2 as for exo_sht_set_tiles, and 3 = half the
- * field-levels with 512-thread blocks.  Variants 2 and 3 run the persistent
+ * group k, 0-3 as for exo_sht_set_tiles.  Variants 2 and 3 run the persistent
  * kernel with 512-thread blocks, so they hold for every group of that
  * kernel or for none (else EXO_ERR_ARG).  set_tiles captures the graph
  * again.  autotune times every combination per truncation (0/1, and 2/3),

@@ -154,13 +154,13 @@ def test_roundtrip_t127_l40_tidal(plans, oracle, gpu, mode):
     # the T127 L40 workload: spectral slope (n+1)^-2 with a wave-1 day-night
     # pattern of amplitude 10 on Re(m=1, n=1) of every field-level; every
     # field of both members within 1e-6 rel L2 of the FP64 oracle, for the
-    # default tiles and the two variants
+    # default tiles and the three variants
     print(f"roundtrip T127 L40 slope 2 wave-1 {mode}", flush=True)
     p = get_plan(plans, 127)
     sp = ensemble(127, 40, 2, cfg=4, slope=2.0, wave1=10.0)
     refs = _oracle_roundtrip(oracle, 127, 40, sp)
     try:
-        for var in (0, 1, 2):
+        for var in (0, 1, 2, 3):
             p.set_tiles(mode, var, var, var)
             _check_roundtrip(p, oracle, mode, 127, 40, nmem=2, cfg=4, slope=2.0, wave1=10.0,
                              refs=refs)
@@ -269,7 +269,7 @@ def test_inverse_all(plans, oracle, gpu, mode, ntru, nlev):
     g_ref = np.stack([oracle.roundtrip(ntru, nlev, sp[k])[0] for k in range(nmem)])
     sp_in, gp, _ = p.alloc_roundtrip(nmem, nlev, mode)
     sp_in.copy_(torch.from_numpy(sp).to(sp_in.dtype))
-    for var in (0, 1, 2):
+    for var in (0, 1, 2, 3):
         p.set_tiles(mode, var, 0, 0)
         gp.fill_(float("nan"))
         p.inverse_all(mode, sp_in, gp, nlev)
@@ -286,10 +286,10 @@ def test_autotune(oracle, gpu):
     p = sht.Plan(42)
     try:
         v = p.autotune("bf16x3", 16, 20)
-        assert all(x in (0, 1, 2) for x in v) and p.tiles("bf16x3") == v
+        assert all(x in (0, 1, 2, 3) for x in v) and p.tiles("bf16x3") == v
         _check_roundtrip(p, oracle, "bf16x3", 42, 20)
         with pytest.raises(sht.ExoError):
-            p.set_tiles("bf16x3", 3, 0, 0)
+            p.set_tiles("bf16x3", 4, 0, 0)
     finally:
         p.close()
 
