@@ -138,3 +138,55 @@ subroutine fc2gp(a, n, lot)
     call exo_check(exo_fc2gp_f32(c_loc(a(1)), int(n, c_int), int(lot, c_int)), 'fc2gp')
   endif
 end subroutine fc2gp
+
+! fftini (src/fftmod.f90:93-122, fft991mod.f90:87-112): the check of the
+! FFT length.  The twiddles live in the GPU plan (built by legini), so only
+! the check remains: an unsupported n ends the run through the abort path,
+! as the reference's stop does.
+subroutine fftini(n)
+  use exo_legmod_capi
+  implicit none
+  integer :: n
+  character(len=80) :: msg
+  if (n /= 64 .and. n /= 128 .and. n /= 192 .and. n /= 384) then
+    write(msg, '(a,i0,a)') 'exoplasim_amd: FFT does not support n = ', n, &
+        ' (NLON 64, 128, 192, 384)'
+    call exo_abort(msg)
+  endif
+end subroutine fftini
+
+! ql (src/gaussmod.f90:46-69): the Legendre polynomial of degree k at p,
+! normalised to a unit integral of its square over [-1, 1], by the upward
+! three-term recurrence of the normalised polynomials
+real(kind=8) function ql(k, p)
+  implicit none
+  integer :: k
+  real(kind=8) :: p
+  real(kind=8) :: z0, z1, z2
+  integer :: j
+  z0 = sqrt(0.5_8)
+  if (k == 0) then
+    ql = z0
+    return
+  endif
+  z1 = sqrt(1.5_8) * p
+  do j = 2, k
+    z2 = sqrt((2.0_8 * j + 1.0_8) * (2.0_8 * j - 1.0_8)) / j * p * z1 &
+         - (j - 1.0_8) / j * sqrt((2.0_8 * j + 1.0_8) / (2.0_8 * j - 3.0_8)) * z0
+    z0 = z1
+    z1 = z2
+  enddo
+  ql = z1
+end function ql
+
+! qld (src/gaussmod.f90:75-86): the reciprocal of dql/dp, the Newton factor
+! of inigau, from (p^2 - 1) dP_k/dp = k (p P_k - c_k P_{k-1}) for the
+! normalised polynomials, c_k = sqrt((2k+1)/(2k-1))
+real(kind=8) function qld(k, p)
+  implicit none
+  integer :: k
+  real(kind=8) :: p
+  real(kind=8), external :: ql
+  qld = (p * p - 1.0_8) / (k * (p * ql(k, p) - sqrt((2.0_8 * k + 1.0_8) / &
+        (2.0_8 * k - 1.0_8)) * ql(k - 1, p)))
+end function qld
