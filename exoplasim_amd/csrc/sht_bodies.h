@@ -34,6 +34,16 @@ struct InvCfg {
   // within four waves per SIMD)
   static constexpr int D1 = 2;
   static constexpr int D2 = 1;
+  // Latitude-pair blocks per tile of the block order (scalar, dv2uv items):
+  // a member's items run as PBT pair blocks x all its field groups, then the
+  // next PBT pair blocks.  An XCD runs about 32 consecutive items at once,
+  // which then read PBT table slices (4 x 442 KB at T127) instead of all
+  // NPAIR / LP of them (5.3 MB per table at T127, above the XCD's 4 MB L2,
+  // which a field-group-major order fetches again for every group).  Below
+  // T127 all slices of a table fit in L2 together, and the order is field
+  // group major (PBT = NPAIR / LP).
+  static constexpr int PBT_SC = (T == 127) ? 4 : TS<T>::NPAIR / LP;
+  static constexpr int PBT_UV = (T == 127) ? 2 : TS<T>::NPAIR / LP;
 };
 
 template <int T, int MODE, int V = 0>
@@ -105,10 +115,18 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
 
   const int tid = item_tid();
   const int ngroups = (KIND == INV_UV) ? (a.nlev + FB / 2 - 1) / (FB / 2) : (a.nfield + FB - 1) / FB;
-  const int pb = blk % NPB;
-  const int g = (blk / NPB) % ngroups;
+  // blk -> (member, pair block, field group): per member, tiles of PBT pair
+  // blocks, each tile running its PBT pair blocks of field group 0, then of
+  // group 1, ...
+  constexpr int PBT =
+      (KIND == INV_UV) ? InvCfg<T, MODE, V>::PBT_UV : InvCfg<T, MODE, V>::PBT_SC;
+  static_assert(NPB % PBT == 0, "pair-block tiles must divide the pair blocks");
   const int mem = blk / (NPB * ngroups);
   if (mem >= a.nmem) return;
+  const int loc = blk - mem * (NPB * ngroups);
+  const int tile = loc / (PBT * ngroups), rt = loc - tile * (PBT * ngroups);
+  const int g = rt / PBT;
+  const int pb = tile * PBT + (rt - g * PBT);
   const R* spm = (const R*)a.sp + mem * a.sp_stride;
   R* outm = (R*)a.out + mem * a.out_stride;
 
@@ -339,9 +357,13 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     struct Item {
       int m, kb;
     };
+    // (read back as wave-uniform scalars: a select that the compiler makes
+    // per lane would turn the table offsets into vector values and each load
+    // into a loop over lanes)
     auto next_item = [&](Item it) -> Item {
       const int kbn = (int)((unsigned)(NM - it.m + 31) >> 5);  // m <= T + NW
-      return (it.kb + 1 < kbn) ? Item{it.m, it.kb + 1} : Item{it.m + NW, 0};
+      const Item n = (it.kb + 1 < kbn) ? Item{it.m, it.kb + 1} : Item{it.m + NW, 0};
+      return Item{__builtin_amdgcn_readfirstlane(n.m), __builtin_amdgcn_readfirstlane(n.kb)};
     };
     const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
     f32x4 acc1 = zero4, acc1b = zero4, acc2 = zero4, acc2b = zero4;
@@ -418,15 +440,36 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     constexpr int DEPTH = (NTERM == 1) ? InvCfg<T, MODE, V>::D1 : InvCfg<T, MODE, V>::D2;
     Frag fr[DEPTH + 1];
     Item cur = {wv, 0}, ahead = cur;
+    // The items ahead load in item order, as the loop's steps load them:
+    // the waits in the loop are counted once for the loop entry and the
+    // back edge together, so a prologue whose last loads were the first
+    // item's would make every pass of the loop wait for all its loads.
 #pragma unroll
     for (int k = 0; k < DEPTH; ++k) {
       fetch(ahead.m, ahead.kb, fr[k].b, fr[k].a0, fr[k].a1);
+      __builtin_amdgcn_sched_barrier(0);
       ahead = next_item(ahead);
     }
-    while (cur.m <= T) {
+    // The wave's item count, so that the loop below runs whole rounds of
+    // DEPTH + 1 items and has no exit between its rounds: its head is then
+    // reached only from its start and its back edge, with the same loads
+    // pending from both, and the waits in it count only the older items.
+    int nit = 0;
+    for (int m = wv; m <= T; m += NW) nit += (int)((unsigned)(NM - m + 31) >> 5);
+    nit = __builtin_amdgcn_readfirstlane(nit);
+    int done = 0;
+    for (; done + DEPTH + 1 <= nit; done += DEPTH + 1) {
 #pragma unroll
       for (int r = 0; r <= DEPTH; ++r) {
-        if (r > 0 && cur.m > T) break;
+        step(cur, fr[r], fr[(r + DEPTH) % (DEPTH + 1)], ahead);
+        cur = next_item(cur);
+        ahead = next_item(ahead);
+      }
+    }
+    // the last nit mod (DEPTH + 1) items, in the same buffer rotation
+#pragma unroll
+    for (int r = 0; r < DEPTH; ++r) {
+      if (done + r < nit) {
         step(cur, fr[r], fr[(r + DEPTH) % (DEPTH + 1)], ahead);
         cur = next_item(cur);
         ahead = next_item(ahead);
