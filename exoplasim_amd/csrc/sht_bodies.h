@@ -25,8 +25,14 @@ using RealT = typename std::conditional<MODE == MODE_F64, double, float>::type;
 template <int T, int MODE, int V = 0>
 struct InvCfg {
   static constexpr int LP = 8;  // latitude pairs per tile (= 16 MFMA rows)
+  // STAGED (T127, MFMA modes): the fused inverse keeps the Legendre sums as
+  // compact Fourier coefficients (m <= T, 16 bytes per row and m: 128 KB for
+  // 8 pairs x 8 fields, all 16 MFMA columns) and each wave then transforms
+  // its rows one at a time in a row buffer of its own.  Full FFT rows of 8
+  // fields (202 KB) would not fit a CU's 160 KB of LDS.
+  static constexpr bool STAGED = (T == 127) && (MODE != MODE_F64);
   static constexpr int FB0 =
-      (T == 127) ? (MODE == MODE_F64 ? 2 : 4) : (MODE == MODE_F64 ? 4 : 8);
+      (T == 127) ? (MODE == MODE_F64 ? 2 : 8) : (MODE == MODE_F64 ? 4 : 8);
   static constexpr int FB = (V == 1 || V == 3) ? ((FB0 >> 1) < 2 ? 2 : (FB0 >> 1)) : FB0;
   static constexpr int NT = (V == 2 || V == 3) ? 512 : 256;  // threads per block
   // MFMA work items in flight per wave, one table (D1) or two (D2; dv2uv
@@ -74,13 +80,19 @@ __device__ __forceinline__ f32x4 mfma_bf16(const BF8& a, const BF8& b, f32x4 c) 
 // x FB fields (north and south).  Fused: the FFT rows of those LP x FB
 // (pair, field) rows, z_k = fc_N(k) + i fc_S(k) with its Hermitian extension,
 // which the Legendre sums write directly, plus the twiddles.
+// Staged (InvCfg::STAGED): the coefficients as unfused, then one padded FFT
+// row per wave.
 template <int T, int MODE, bool FUSED, int V>
 struct InvLds {
   using R = RealT<MODE>;
   using C = InvCfg<T, MODE, V>;
-  static constexpr size_t FCL = FUSED ? 0 : sizeof(NS2<R>) * C::LP * C::FB * TS<T>::NM;
+  static constexpr bool ST = FUSED && C::STAGED;
+  static constexpr size_t FCL =
+      (FUSED && !ST) ? 0 : sizeof(NS2<R>) * C::LP * C::FB * TS<T>::NM;
   static constexpr size_t FBUF =
-      FUSED ? sizeof(C2<R>) * fft_buf_elems(C::LP * C::FB * TS<T>::NLON) : 0;
+      ST      ? sizeof(C2<R>) * (C::NT / 64) * fft_buf_elems(TS<T>::NLON)
+      : FUSED ? sizeof(C2<R>) * fft_buf_elems(C::LP * C::FB * TS<T>::NLON)
+              : 0;
   static constexpr size_t TWS = FUSED ? sizeof(C2<R>) * TS<T>::NLON : 0;
   static constexpr size_t BYTES = FCL + FBUF + TWS;
 };
@@ -107,6 +119,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
   constexpr int NT = InvCfg<T, MODE, V>::NT, NW = NT / 64;  // threads, waves
   constexpr int NTERM = (KIND == INV_UV) ? 2 : 1;
   constexpr int NRP = N + N / 32;  // padded FFT row (fused)
+  constexpr bool ST = FUSED && InvCfg<T, MODE, V>::STAGED;
 
   using L = InvLds<T, MODE, FUSED, V>;
   NS2<R>* fcl = (NS2<R>*)lds;
@@ -134,10 +147,13 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     const C2<R>* tw = (const C2<R>*)twp;
     for (int k = tid; k < N; k += NT) tws[k] = tw[k];
     // FFT inputs T+1 .. N-T-1 (above the truncation, Nyquist included) are 0
+    // (staged: not stored, the FFT's first pass leaves them out)
     constexpr int NZ = N - 2 * T - 1;
-    for (int i = tid; i < LP * FB * NZ; i += NT) {
-      const int f = i / NZ, k = T + 1 + (i - f * NZ);
-      fbuf[f * NRP + k + (k >> 5)] = C2<R>{R(0), R(0)};
+    if constexpr (!ST) {
+      for (int i = tid; i < LP * FB * NZ; i += NT) {
+        const int f = i / NZ, k = T + 1 + (i - f * NZ);
+        fbuf[f * NRP + k + (k >> 5)] = C2<R>{R(0), R(0)};
+      }
     }
   }
 
@@ -146,7 +162,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
   // coefficients of wavenumber m; the imaginary parts of m = 0 are ignored,
   // as fc2gp does.  Unfused: the coefficients as they are.
   auto put = [&](int pl, int fl, int m, NS2<R> e) {
-    if constexpr (FUSED) {
+    if constexpr (FUSED && !ST) {
       C2<R>* row = fbuf + (pl * FB + fl) * NRP;
       if (m == 0) {
         row[0] = C2<R>{e.n.x, e.s.x};
@@ -407,7 +423,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
             nv = ri ? np * fm : -np * fm;
             sv = ri ? spn * fm : -spn * fm;
           }
-          if constexpr (FUSED) {
+          if constexpr (FUSED && !ST) {
             // the partner column (lane ^ 1) holds the other component: sp
             // is s.y for ri = 0 and s.x for ri = 1
             const float sp = quad_perm<0xB1>(sv);  // lane ^ 1
@@ -546,6 +562,49 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
       o[(long)(NLAT - 1 - lat) * N + 2 * m] = vs.x;
       o[(long)(NLAT - 1 - lat) * N + 2 * m + 1] = vs.y;
     }
+  } else if constexpr (ST) {
+    // ---- phase 2 (staged): wave wv transforms rows wv, wv + NW, ... one at
+    // a time in its own row buffer; the first FFT pass builds
+    // z = fc_N + i fc_S with its Hermitian extension from the compact
+    // coefficients as it reads them.  Re -> north row, Im -> south row ----
+    static_assert(N == 384, "the staged inverse runs the 384-point wave FFT");
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    C2<R>* wrow = fbuf + wv * NRP;
+    typedef R R4 __attribute__((ext_vector_type(4)));
+    for (int f = wv; f < LP * FB; f += NW) {
+      const int pl = f / FB, fl = f - pl * FB;
+      const int d = dst_of(fl);
+      if (d < 0) continue;
+      const NS2<R>* e = fcl + f * NM;
+      // input k = lane + 64 r: r = 0, 1 the stored m = k (z_0 = (n.x, s.x),
+      // the imaginary parts of m = 0 ignored as fc2gp does); r = 2, 3 and
+      // k = 256 zero (above the truncation); else z_{N-m}, m = N - k
+      auto load = [&](int r, int k) -> C2<R> {
+        if (r < 2) {
+          const NS2<R> v = e[k];
+          return (k == 0) ? C2<R>{v.n.x, v.s.x} : C2<R>{v.n.x - v.s.y, v.n.y + v.s.x};
+        }
+        if (r < 4 || k < N - T) return C2<R>{R(0), R(0)};
+        const NS2<R> v = e[N - k];
+        return C2<R>{v.n.x + v.s.y, v.s.x - v.n.y};
+      };
+      fft384_wave<+1, R>(wrow, tws, lane, load);
+      // whole rows in runs of 4 values per lane, non-temporal (as below)
+      const int lat = pb * LP + pl;
+      R* o = outm + (long)d * NLAT * N;
+      for (int k = 4 * lane; k < N; k += 256) {
+        const C2<R>* z = wrow + k + (k >> 5);
+        const C2<R> z0 = z[0], z1 = z[1], z2 = z[2], z3 = z[3];
+        __builtin_nontemporal_store(R4{z0.x, z1.x, z2.x, z3.x}, (R4*)(o + (long)lat * N + k));
+        __builtin_nontemporal_store(R4{z0.y, z1.y, z2.y, z3.y},
+                                    (R4*)(o + (long)(NLAT - 1 - lat) * N + k));
+      }
+      // the row is read before the wave's next row rewrites it
+      wave_lds_sync();
+    }
+    // the coefficients are read before a next work item of the block
+    // rewrites them
+    __syncthreads();
   } else {
     // ---- phase 2: inverse FFT of the rows z = fc_N + i fc_S, all LP*FB at
     // once and in place: Re -> north row, Im -> south row ----

@@ -212,6 +212,27 @@ struct Dft<8, S, F> {
   }
 };
 
+// 6 = 3 x 2: DFT3 of the even and the odd inputs, a[k1] and b[k1]; then
+// X[k1] = a[k1] + w6^k1 b[k1] and X[k1 + 3] = a[k1] - w6^k1 b[k1]
+template <int S, typename F>
+struct Dft<6, S, F> {
+  __device__ __forceinline__ static void run(C2<F>* v) {
+    const F h = F(0.86602540378443864676);  // sqrt(3)/2
+    C2<F> a[3] = {v[0], v[2], v[4]};
+    C2<F> b[3] = {v[1], v[3], v[5]};
+    Dft<3, S, F>::run(a);
+    Dft<3, S, F>::run(b);
+    const C2<F> b1 = cmul(b[1], C2<F>{F(0.5), F(S) * h});   // w6
+    const C2<F> b2 = cmul(b[2], C2<F>{F(-0.5), F(S) * h});  // w6^2
+    v[0] = cadd(a[0], b[0]);
+    v[3] = csub(a[0], b[0]);
+    v[1] = cadd(a[1], b1);
+    v[4] = csub(a[1], b1);
+    v[2] = cadd(a[2], b2);
+    v[5] = csub(a[2], b2);
+  }
+};
+
 // 16 = 4 x 4: DFT4 over n1 of v[4 n1 + n2], twiddle w16^(n2 k1), DFT4 over
 // n2 into X[k1 + 4 k2]
 template <int S, typename F>
@@ -481,6 +502,64 @@ __device__ __forceinline__ void fft_run_lds(C2<F>* buf, const C2<F>* tw, Store& 
     fft_pass<N, R3, R0 * R1 * R2, S, false, OUT, F, NT, NFM, ZLO, ZHI>(buf, tw, nf, none,
                                                                       store);
   }
+}
+
+// The LDS writes of one wave before this point are seen by the LDS reads of
+// its other lanes after it: a wave's LDS instructions execute in order, so
+// only the compiler must keep them in program order.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// FFT of one row of 384 complex values by one wave (lane = 0..63), with
+// passes of radix 6, 8 and 8 (Stockham autosort; the first pass has one
+// butterfly per lane).  The first pass reads input k through load(k); the
+// transform is left in `row` (384 + 12 complex, padded as bix).  tw[k] =
+// e^{S 2 pi i k / 384}.  Only the calling wave touches `row`.
+template <int S, typename F, typename Load>
+__device__ __forceinline__ void fft384_wave(C2<F>* row, const C2<F>* tw, int lane, Load& load) {
+  {  // radix 6, stride 1: butterfly j = lane takes k = j + 64 r
+    C2<F> v[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) v[r] = load(r, lane + 64 * r);
+    Dft<6, S, F>::run(v);
+#pragma unroll
+    for (int r = 0; r < 6; ++r) row[bix(6 * lane + r)] = v[r];
+  }
+  wave_lds_sync();
+  // radix 8, stride 6, then radix 8, stride 48: butterflies j < 48 take
+  // k = j + 48 r; in place, all reads of the wave before its writes
+  C2<F> v[8];
+  if (lane < 48) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = row[bix(lane + 48 * r)];
+  }
+  wave_lds_sync();
+  if (lane < 48) {
+    const int jm = lane % 6;
+#pragma unroll
+    for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], tw[jm * r * 8]);
+    Dft<8, S, F>::run(v);
+    const int o = (lane / 6) * 48 + jm;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) row[bix(o + 6 * r)] = v[r];
+  }
+  wave_lds_sync();
+  if (lane < 48) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = row[bix(lane + 48 * r)];
+  }
+  wave_lds_sync();
+  if (lane < 48) {
+#pragma unroll
+    for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], tw[lane * r]);
+    Dft<8, S, F>::run(v);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) row[bix(lane + 48 * r)] = v[r];
+  }
+  wave_lds_sync();
 }
 
 // ---- split-BF16 helpers ----
