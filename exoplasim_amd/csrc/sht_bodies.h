@@ -377,7 +377,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     // per lane would turn the table offsets into vector values and each load
     // into a loop over lanes)
     auto next_item = [&](Item it) -> Item {
-      const int kbn = (int)((unsigned)(NM - it.m + 31) >> 5);  // m <= T + NW
+      const int kbn = (NM - it.m + 31) >> 5;  // <= 0 past the truncation
       const Item n = (it.kb + 1 < kbn) ? Item{it.m, it.kb + 1} : Item{it.m + NW, 0};
       return Item{__builtin_amdgcn_readfirstlane(n.m), __builtin_amdgcn_readfirstlane(n.kb)};
     };
@@ -392,7 +392,9 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
       split(use.b, bp);
       mul(use.b, bp, use.a0, acc1, acc1b);
       if (NTERM == 2) mul(use.b, bp, use.a1, acc2, acc2b);
-      if (next_item(cur).kb == 0) {
+      // (a step past the wave's last item multiplies the zeros it loaded
+      // and stores nothing)
+      if (next_item(cur).kb == 0 && cur.m <= T) {
         const int m = cur.m;
         acc1 += acc1b;
         if (NTERM == 2) acc2 += acc2b;
@@ -466,26 +468,17 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
       __builtin_amdgcn_sched_barrier(0);
       ahead = next_item(ahead);
     }
-    // The wave's item count, so that the loop below runs whole rounds of
-    // DEPTH + 1 items and has no exit between its rounds: its head is then
-    // reached only from its start and its back edge, with the same loads
-    // pending from both, and the waits in it count only the older items.
+    // Whole rounds of DEPTH + 1 steps, the steps past the wave's last item
+    // only issuing their (empty) loads: the loop has no exit between its
+    // rounds, so its head is reached only from its start and its back edge,
+    // with the same loads pending from both, and the waits in it count only
+    // the older items.
     int nit = 0;
-    for (int m = wv; m <= T; m += NW) nit += (int)((unsigned)(NM - m + 31) >> 5);
-    nit = __builtin_amdgcn_readfirstlane(nit);
-    int done = 0;
-    for (; done + DEPTH + 1 <= nit; done += DEPTH + 1) {
+    for (int m = wv; m <= T; m += NW) nit += (NM - m + 31) >> 5;
+    const int nround = __builtin_amdgcn_readfirstlane((nit + DEPTH) / (DEPTH + 1));
+    for (int rd = 0; rd < nround; ++rd) {
 #pragma unroll
       for (int r = 0; r <= DEPTH; ++r) {
-        step(cur, fr[r], fr[(r + DEPTH) % (DEPTH + 1)], ahead);
-        cur = next_item(cur);
-        ahead = next_item(ahead);
-      }
-    }
-    // the last nit mod (DEPTH + 1) items, in the same buffer rotation
-#pragma unroll
-    for (int r = 0; r < DEPTH; ++r) {
-      if (done + r < nit) {
         step(cur, fr[r], fr[(r + DEPTH) % (DEPTH + 1)], ahead);
         cur = next_item(cur);
         ahead = next_item(ahead);
