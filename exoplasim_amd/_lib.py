@@ -78,6 +78,8 @@ PROTOTYPES = {
     "exo_sht_ens_get_tiles": (c_int, [c_void_p, POINTER(c_int)]),
     "exo_sht_ens_set_tiles": (c_int, [c_void_p, POINTER(c_int)]),
     "exo_sht_ens_autotune": (c_int, [c_void_p, c_void_p, POINTER(c_int)]),
+    "exo_sht_ens_get_schedule": (c_int, [c_void_p, POINTER(c_int)]),
+    "exo_sht_ens_set_schedule": (c_int, [c_void_p, c_int]),
     "exo_si_create": (c_int, [POINTER(c_void_p), c_int, c_int, _dp, _dp, _dp, _dp]),
     "exo_si_destroy": (c_int, [c_void_p]),
     "exo_si_makebm": (c_int, [c_void_p, c_double, c_void_p]),

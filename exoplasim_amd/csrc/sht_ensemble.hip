@@ -253,6 +253,9 @@ struct exo_sht_ens_s {
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // what the graph holds: the two persistent kernels (EXO_ENS_PERSISTENT)
+  // or one batched round trip per group (EXO_ENS_BATCHED)
+  int sched = EXO_ENS_PERSISTENT;
 };
 
 // the per-truncation kernels run the same variants
@@ -466,9 +469,24 @@ static int ens_pack(exo_sht_ens_s* e, hipStream_t s) {
   return EXO_OK;
 }
 
-// the whole ensemble round trip: reset counters, pre-split the inputs, all
-// inverse items, all direct items
+// one batched round trip per group, with the group's tile variants
+static int ens_groups(exo_sht_ens_s* e, hipStream_t s) {
+  for (int k = 0; k < e->ngroup; ++k) {
+    const exo_sht_ens_group& g = e->g[k];
+    int rc = exo_sht_set_tiles(e->plan[k], e->mode, plan_var(e->inv_var[k]),
+                               plan_var(e->inv_var[k]), plan_var(e->fwd_var[k]));
+    if (rc == EXO_OK)
+      rc = exo_sht_roundtrip(e->plan[k], e->mode, g.nmem, g.nlev, g.sp_in, g.gp, g.sp_out, s);
+    if (rc != EXO_OK) return rc;
+  }
+  return EXO_OK;
+}
+
+// the whole ensemble round trip in the chosen schedule: persistent, reset
+// the counters, pre-split the inputs, all inverse items, all direct items;
+// batched, one round trip per group
 static int ens_enqueue(exo_sht_ens_s* e, hipStream_t s) {
+  if (e->sched == EXO_ENS_BATCHED) return ens_groups(e, s);
   k_ens_reset<<<1, 64, 0, s>>>(e->counter);
   ENS_HIPCK(hipGetLastError());
   int rc = ens_pack(e, s);
@@ -486,6 +504,13 @@ static void ens_drop_graph(exo_sht_ens_s* e) {
 
 static int ens_capture(exo_sht_ens_s* e) {
   ens_drop_graph(e);
+  if (e->sched == EXO_ENS_BATCHED) {
+    // one eager pass first: the plans size their scratch on first use,
+    // which must not happen inside the capture
+    const int rc0 = ens_groups(e, e->cap);
+    if (rc0 != EXO_OK) return rc0;
+    ENS_HIPCK(hipStreamSynchronize(e->cap));
+  }
   ENS_HIPCK(hipStreamBeginCapture(e->cap, hipStreamCaptureModeThreadLocal));
   const int rc = ens_enqueue(e, e->cap);
   hipGraph_t g = nullptr;
@@ -578,15 +603,20 @@ extern "C" int exo_sht_ens_run_eager(exo_sht_ens_t e, void* stream) {
 // the same round trips as one batched round trip per group (4 launches each)
 extern "C" int exo_sht_ens_run_groups(exo_sht_ens_t e, void* stream) {
   if (!e) return EXO_ERR_ARG;
-  for (int k = 0; k < e->ngroup; ++k) {
-    const exo_sht_ens_group& g = e->g[k];
-    int rc = exo_sht_set_tiles(e->plan[k], e->mode, plan_var(e->inv_var[k]), plan_var(e->inv_var[k]),
-                               plan_var(e->fwd_var[k]));
-    if (rc == EXO_OK)
-      rc = exo_sht_roundtrip(e->plan[k], e->mode, g.nmem, g.nlev, g.sp_in, g.gp, g.sp_out, stream);
-    if (rc != EXO_OK) return rc;
-  }
+  return ens_groups(e, (hipStream_t)stream);
+}
+
+extern "C" int exo_sht_ens_get_schedule(exo_sht_ens_t e, int* sched) {
+  if (!e || !sched) return EXO_ERR_ARG;
+  *sched = e->sched;
   return EXO_OK;
+}
+
+extern "C" int exo_sht_ens_set_schedule(exo_sht_ens_t e, int sched) {
+  if (!e || (sched != EXO_ENS_PERSISTENT && sched != EXO_ENS_BATCHED)) return EXO_ERR_ARG;
+  (void)hipDeviceSynchronize();
+  e->sched = sched;
+  return ens_capture(e);
 }
 
 // the same round trips one member at a time (4 launches per member)
@@ -647,7 +677,8 @@ extern "C" int exo_sht_ens_set_tiles(exo_sht_ens_t e, const int* vars) {
 // Times each combination of tile variants per truncation (the groups of one
 // truncation share a variant: 0 or 1 with 256-thread blocks, 2 or 3 with
 // 512-thread blocks) for the inverse and the direct kernel, keeps the
-// fastest and captures the graph again.
+// fastest, then times the two schedules with those variants and captures
+// the faster in the graph.
 extern "C" int exo_sht_ens_autotune(exo_sht_ens_t e, void* stream, int* vars) {
   if (!e) return EXO_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
@@ -691,8 +722,6 @@ extern "C" int exo_sht_ens_autotune(exo_sht_ens_t e, void* stream, int* vars) {
       }
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   for (int k = 0; k < e->ngroup; ++k) {
     int bi = 0, bf = 0;
     for (int t = 0; t < 3; ++t)
@@ -704,6 +733,25 @@ extern "C" int exo_sht_ens_autotune(exo_sht_ens_t e, void* stream, int* vars) {
     e->fwd_var[k] = (pick[1] >> 3) * 2 + bf;
   }
   if (rc == EXO_OK) rc = ens_prepare(e);
+  // the whole round trip in each schedule (the first pass of each sizes the
+  // plans' scratch and is not timed)
+  float tsched[2] = {1e30f, 1e30f};
+  for (int sc = 0; sc < 2 && rc == EXO_OK; ++sc) {
+    e->sched = sc;
+    for (int rep = 0; rep < 4 && rc == EXO_OK; ++rep) {
+      (void)hipEventRecord(e0, s);
+      rc = ens_enqueue(e, s);
+      (void)hipEventRecord(e1, s);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (rep > 0) tsched[sc] = std::min(tsched[sc], ms);
+    }
+  }
+  e->sched = tsched[EXO_ENS_BATCHED] < tsched[EXO_ENS_PERSISTENT] ? EXO_ENS_BATCHED
+                                                                   : EXO_ENS_PERSISTENT;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (rc == EXO_OK) rc = ens_capture(e);
   if (rc == EXO_OK && vars) exo_sht_ens_get_tiles(e, vars);
   return rc;

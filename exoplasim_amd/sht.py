@@ -473,8 +473,10 @@ class Ensemble:
     groups: list of (ntru, nlev, nmem) with ntru in 21, 42, 63.  The device
     buffers of each group are allocated here in the layout of
     Plan.roundtrip (``self.bufs[k] = (sp_in, gp, sp_out)``); all round trips
-    run as two persistent kernels captured in one hipGraph
-    (include/exo_sht.h, exo_sht_ens_*).  mode: "f32", "f32mfma" or "bf16x3".
+    run as two persistent kernels captured in one hipGraph, or as one
+    batched round trip per group in that graph (set_schedule; autotune picks
+    the faster) (include/exo_sht.h, exo_sht_ens_*).  mode: "f32", "f32mfma"
+    or "bf16x3".
     """
 
     def __init__(self, groups, mode="bf16x3"):
@@ -538,6 +540,28 @@ class Ensemble:
         check(lib().exo_sht_ens_set_tiles(self._h, v), "ens_set_tiles")
 
     def autotune(self):
+        """Choose the tile variants, then the faster schedule (persistent
+        kernels or batched round trips) for the graph; returns the variants."""
         v = (c_int * (2 * len(self.groups)))()
         check(lib().exo_sht_ens_autotune(self._h, _stream(), v), "ens_autotune")
         return tuple(v)
+
+    PERSISTENT, BATCHED = 0, 1
+
+    def schedule(self) -> str:
+        """What the graph holds: "persistent" (the two persistent kernels) or
+        "batched" (one batched round trip per group)."""
+        v = c_int()
+        check(lib().exo_sht_ens_get_schedule(self._h, byref(v)), "ens_get_schedule")
+        return "batched" if v.value == self.BATCHED else "persistent"
+
+    def set_schedule(self, sched: str):
+        """Capture the graph again with "persistent" or "batched" (the latter
+        after one direct pass, which writes the outputs)."""
+        code = {"persistent": self.PERSISTENT, "batched": self.BATCHED}.get(sched)
+        if code is None:
+            raise ValueError(f"schedule {sched!r}: 'persistent' or 'batched'")
+        import torch
+
+        torch.cuda.synchronize()
+        check(lib().exo_sht_ens_set_schedule(self._h, code), "ens_set_schedule")

@@ -242,8 +242,9 @@ int exo_sht_autotune(exo_sht_plan_t plan, int mode, int nmem, int nlev, void *st
  * (nmem, 5*nlev+2, NLAT, NLON), float device arrays).  Truncations 21, 42
  * and 63; modes EXO_F32, EXO_F32MFMA and EXO_BF16X3.  The round trips of all
  * members run as two persistent kernels (all inverse work items, then all
- * direct work items) captured in one hipGraph at create time; the arrays
- * must stay allocated while the ensemble exists. */
+ * direct work items) captured in one hipGraph at create time, or, after
+ * exo_sht_ens_set_schedule or autotune, as one batched round trip per group
+ * in that graph; the arrays must stay allocated while the ensemble exists. */
 typedef struct {
   int ntru, nlev, nmem;
   const void *sp_in;
@@ -271,6 +272,17 @@ int exo_sht_ens_run_members(exo_sht_ens_t ens, void *stream);
 int exo_sht_ens_get_tiles(exo_sht_ens_t ens, int *vars);
 int exo_sht_ens_set_tiles(exo_sht_ens_t ens, const int *vars);
 int exo_sht_ens_autotune(exo_sht_ens_t ens, void *stream, int *vars);
+/* What the graph of exo_sht_ens_run holds (and exo_sht_ens_run_eager
+ * launches): the two persistent kernels (EXO_ENS_PERSISTENT, the default)
+ * or one batched exo_sht_roundtrip per group (EXO_ENS_BATCHED, the kernels
+ * of exo_sht_ens_run_groups).  set_schedule captures the graph again (for
+ * EXO_ENS_BATCHED after one direct pass, which writes the outputs);
+ * autotune, after choosing the tile variants, times both schedules and
+ * keeps the faster. */
+#define EXO_ENS_PERSISTENT 0
+#define EXO_ENS_BATCHED 1
+int exo_sht_ens_get_schedule(exo_sht_ens_t ens, int *sched);
+int exo_sht_ens_set_schedule(exo_sht_ens_t ens, int sched);
 
 #ifdef __cplusplus
 }

@@ -51,6 +51,10 @@ def test_strerror_and_argument_errors():
     assert lib.exo_sht_plan_destroy(None) == -1
     assert lib.exo_legini(30, 48, 10, None, None, None, None, 1.0) == -2
     assert lib.exo_legini(42, 64, 0, None, None, None, None, 1.0) == -2
+    # ensemble schedule: no ensemble, no output
+    v = ctypes.c_int()
+    assert lib.exo_sht_ens_get_schedule(None, ctypes.byref(v)) == -1
+    assert lib.exo_sht_ens_set_schedule(None, 0) == -1
 
 
 def test_no_fallback_without_library(monkeypatch, tmp_path):

@@ -1,6 +1,6 @@
 """Mixed-truncation ensembles (T21 / T42 / T63 in one hipGraph of two
-persistent kernels) against the FP64 oracle, the per-group batched round trip
-and the one-member-at-a-time loop.
+persistent kernels, or of one batched round trip per group) against the FP64
+oracle, the per-group batched round trip and the one-member-at-a-time loop.
 
 Each work item of the persistent kernels is the same source as one block of
 the per-truncation kernels; compiled into different kernels the FMA
@@ -88,6 +88,41 @@ def test_ensemble_vs_oracle_and_loops(oracle, gpu, mode):
         ens.close()
 
 
+@pytest.mark.parametrize("mode", ["bf16x3", "f32mfma"])
+def test_ensemble_batched_schedule(oracle, gpu, mode):
+    # the graph holding one batched round trip per group: every replay (with
+    # eager work between) bitwise equal to the eager batched launches, and
+    # back to the persistent kernels bitwise equal to their first result
+    print(f"ensemble batched schedule {GROUPS} {mode}", flush=True)
+    ens = sht.Ensemble(GROUPS, mode)
+    try:
+        host = _fill(ens)
+        assert ens.schedule() == "persistent"
+        ens.run()
+        pers = _outputs(ens)
+        _poison(ens)
+        ens.run_groups()
+        grp = _outputs(ens)
+        ens.set_schedule("batched")
+        assert ens.schedule() == "batched"
+        for run in (ens.run, ens.run_groups, ens.run, ens.run_eager, ens.run):
+            _poison(ens)
+            run()
+            assert _same(_outputs(ens), grp), run.__name__
+        for k, (ntru, nlev, nmem) in enumerate(ens.groups):
+            for m in range(nmem):
+                g_o, s_o = oracle.roundtrip(ntru, nlev, host[k][m])
+                assert rel_l2(grp[k][0][m], g_o) < 1e-6 and rel_l2(grp[k][1][m], s_o) < 1e-6
+        ens.set_schedule("persistent")
+        _poison(ens)
+        ens.run()
+        assert _same(_outputs(ens), pers)
+        with pytest.raises(ValueError):
+            ens.set_schedule("fused")
+    finally:
+        ens.close()
+
+
 def test_ensemble_tiles_and_autotune(oracle, gpu):
     print("ensemble tiles/autotune", flush=True)
     ens = sht.Ensemble(GROUPS, "bf16x3")
@@ -123,6 +158,7 @@ def test_ensemble_tiles_and_autotune(oracle, gpu):
             ens.set_tiles([2, 0] + [0] * (2 * len(GROUPS) - 2))
         v = ens.autotune()
         assert all(x in (0, 1, 2, 3) for x in v) and ens.tiles() == v
+        assert ens.schedule() in ("persistent", "batched")
         for w in (0, 1):  # 512-thread variants for all groups of a kernel or for none
             assert len({x >= 2 for x in v[w::2]}) == 1
         # groups of one truncation share a variant
