@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void k_dirx(DirxArgs a, const void* tabv, void
 // (tables), more pairs per step.
 template <int T, int NIN>
 struct DirxMfCfg {
-  static constexpr int MR = (T == 63) ? 32 : (T == 127) ? 16 : 22;  // wavenumbers per block
+  static constexpr int MR = 8;  // wavenumbers per block
   static constexpr int NMR = (TS<T>::NM + MR - 1) / MR;
   static constexpr int NT = 512, NW = NT / 64;
   static constexpr int MPW = (MR + NW - 1) / NW;                     // wavenumbers per wave
