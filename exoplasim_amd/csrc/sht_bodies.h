@@ -208,8 +208,10 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
   };
 
   // ---- phase 1: Legendre sums, into the FFT rows (fused) or fcl ----
-  if constexpr (MODE == MODE_BF16X3 || MODE == MODE_F32MFMA) {
+  if constexpr (MODE == MODE_BF16X3 || MODE == MODE_F32MFMA || MODE == MODE_F64) {
     // GEMM per m: C[row = 2*pair + class][col] = sum_n A[row][n] B[n][col];
+    // split-BF16 on v_mfma_f32_16x16x32_bf16, FP32 on v_mfma_f32_16x16x4_f32,
+    // FP64 on v_mfma_f64_16x16x4_f64 (the FP32 operand layouts in doubles);
     // wave wv of NW takes m = wv, wv + NW, ...; each (m, 32-wide n block) is
     // one work item, and the operands of the items ahead are fetched before
     // the current one is multiplied.
@@ -220,7 +222,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     const int q = lane >> 4, c = lane & 15;
     // column c of B: SCALAR field g*FB + c/2; UV level g*FB/2 + c/4 of
     // vorticity ((c/2)&1 == 0) or divergence; component c&1
-    const R* bp = nullptr;  // F32MFMA: the column's fp32 coefficients
+    const R* bp = nullptr;  // F32MFMA, F64: the column's coefficients
     bool isvor = false;
     int pf = -1;  // field of column c
     if (KIND == INV_SCALAR) {
@@ -234,7 +236,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
         isvor = (s == 0) && ((c & 1) == 0);
       }
     }
-    if constexpr (MODE == MODE_F32MFMA) {
+    if constexpr (MODE != MODE_BF16X3) {
       if (pf >= 0) bp = spm + (long)pf * S::NRSP + (c & 1);
     }
     // Operands through buffer loads: a per-lane byte offset plus a
@@ -243,11 +245,11 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     // element j = 16 kb + 8 (q & 1) .. + 7 (layouts in sht_common.h).
     // A: the table rows of pair block pb; a lane whose parity is not its
     // row's class, or whose j lies past the row, reads past the buffer,
-    // which returns the zeros.  F32MFMA: item (m, kb) at ((m KB + kb) 512)
-    // floats of the block's table.
-    constexpr int AE = (MODE == MODE_BF16X3) ? 2 : 4;  // bytes per table element
+    // which returns the zeros.  F32MFMA, F64: item (m, kb) at ((m KB + kb)
+    // 512) elements of the block's table.
+    constexpr int AE = (MODE == MODE_BF16X3) ? 2 : (int)sizeof(R);  // bytes per table element
     const long abase = (MODE == MODE_BF16X3) ? (long)pb * tab_pb(T) : (long)pb * NM * S::KB * 512;
-    const int abytes = (MODE == MODE_BF16X3) ? tab_pb(T) * 2 : (int)(plane - abase) * 4;
+    const int abytes = (MODE == MODE_BF16X3) ? tab_pb(T) * 2 : (int)(plane - abase) * AE;
     const __amdgpu_buffer_rsrc_t rsA0 = buffer_rsrc((const char*)tabA0 + abase * AE, abytes);
     const __amdgpu_buffer_rsrc_t rsA1 =
         buffer_rsrc((const char*)(NTERM == 2 ? tabA1 : tabA0) + abase * AE, abytes);
@@ -255,7 +257,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     // does this lane hold nonzero table values (its parity is its row's class)
     const bool amine0 = hq == ((c & 1) ^ (table_is_q(KIND == INV_UV ? TQV : a.table) ? 1 : 0));
     const bool amine1 = hq == ((c & 1) ^ (table_is_q(TQU) ? 1 : 0));
-    const int avoff = (q * 16 + c) * 32;  // F32MFMA
+    const int avoff = (q * 16 + c) * 8 * AE;  // F32MFMA, F64: 8 values per lane
     // B (BF16X3): the member's parity-ordered fp32 operand; columns without
     // a field read past the buffer, which returns zeros
     __amdgpu_buffer_rsrc_t rsB = rsA0;
@@ -266,15 +268,18 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     }
     const int brow = (c & 1) * 2 + hq;  // re/im, parity row of the B segment
     const R pv = (R)a.plavor;
-    // B fragment of one item: 8 fp32 values (F32MFMA, n = 4j + q), or 8
+    // B fragment of one item: 8 values (F32MFMA, F64: f, n = 4j + q), or 8
     // fp32 values of one parity as loaded (BF16X3: r, split into the three
     // bf16 planes when the item is multiplied; fix marks the lane that holds
     // the absolute vorticity at (m=0, n=1, re))
     struct BFrag {
-      float f[8];
+      R f[8];
       uint4 r[2];
       bool fix;
     };
+    // 16-byte table loads per item and term: three bf16 planes, or 8 values
+    // of 4 or 8 bytes
+    constexpr int NA = (MODE == MODE_BF16X3) ? 3 : (8 * AE) / 16;
     // Every step fetches, also past the last item (m > T: nothing is loaded,
     // at the offsets of m = T), so that each step issues the same loads and
     // the wait for an item's operands counts only the loads issued before
@@ -317,15 +322,15 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
             x = bp[2 * (w0 + n - m)];
             if (isvor && m == 0 && n == 1) x = x - pv;
           }
-          br.f[j] = (float)x;
+          br.f[j] = x;
         }
-        // f32 operand stored as [q][row][8] so one lane reads 8 floats
+        // operand stored as [q][row][8] so one lane reads its 8 values
         const int aso = (m * S::KB + kb) * 512 * AE;
-        a0[0] = buffer_load_x4(rsA0, avoff, aso);
-        a0[1] = buffer_load_x4(rsA0, avoff, aso + 16);
+#pragma unroll
+        for (int i = 0; i < NA; ++i) a0[i] = buffer_load_x4(rsA0, avoff, aso + 16 * i);
         if (NTERM == 2) {
-          a1[0] = buffer_load_x4(rsA1, avoff, aso);
-          a1[1] = buffer_load_x4(rsA1, avoff, aso + 16);
+#pragma unroll
+          for (int i = 0; i < NA; ++i) a1[i] = buffer_load_x4(rsA1, avoff, aso + 16 * i);
         }
       }
     };
@@ -344,7 +349,8 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     // two independent accumulation chains per term (the small cross terms
     // and the large ones), so that consecutive MFMAs do not all wait for the
     // previous result
-    auto mul = [&](const BFrag& br, const BF8* bp, const uint4* a, f32x4& acc, f32x4& acc2) {
+    using Acc = typename std::conditional<MODE == MODE_F64, f64x4, f32x4>::type;
+    auto mul = [&](const BFrag& br, const BF8* bp, const uint4* a, Acc& acc, Acc& acc2) {
       if constexpr (MODE == MODE_BF16X3) {
         BF8 ah, am, al;
         ah.q = a[0];
@@ -356,6 +362,13 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
         acc2 = mfma_bf16(ah, bp[1], acc2);
         acc = mfma_bf16(al, bp[0], acc);
         acc2 = mfma_bf16(ah, bp[0], acc2);
+      } else if constexpr (MODE == MODE_F64) {
+        const double* ad = (const double*)a;
+#pragma unroll
+        for (int j = 0; j < 8; j += 2) {
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ad[j], br.f[j], acc, 0, 0, 0);
+          acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ad[j + 1], br.f[j + 1], acc2, 0, 0, 0);
+        }
       } else {
         const float* af = (const float*)a;
 #pragma unroll
@@ -368,7 +381,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     // operands of one work item (m, kb)
     struct Frag {
       BFrag b;
-      uint4 a0[3], a1[3];
+      uint4 a0[NA], a1[NA];
     };
     struct Item {
       int m, kb;
@@ -381,8 +394,8 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
       const Item n = (it.kb + 1 < kbn) ? Item{it.m, it.kb + 1} : Item{it.m + NW, 0};
       return Item{__builtin_amdgcn_readfirstlane(n.m), __builtin_amdgcn_readfirstlane(n.kb)};
     };
-    const f32x4 zero4 = {0.f, 0.f, 0.f, 0.f};
-    f32x4 acc1 = zero4, acc1b = zero4, acc2 = zero4, acc2b = zero4;
+    const Acc zero4 = {0, 0, 0, 0};
+    Acc acc1 = zero4, acc1b = zero4, acc2 = zero4, acc2b = zero4;
     // multiply item `cur` (operands in `use`) after starting the loads of
     // item `ahead` (two items ahead) into `into`; at the last n block of an
     // m, combine the terms, fold N/S and store the Fourier coefficients
@@ -398,15 +411,15 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
         const int m = cur.m;
         acc1 += acc1b;
         if (NTERM == 2) acc2 += acc2b;
-        f32x4 out = acc1;
+        Acc out = acc1;
         if (KIND == INV_UV) {
           // u = qv.zeta + qu.(-i D), v = qu.(-i zeta) - qv.D: the qu product of
           // the partner column (zeta.re <-> D.im, zeta.im <-> D.re)
-          f32x4 p2;
+          Acc p2;
 #pragma unroll
           for (int i = 0; i < 4; ++i) p2[i] = quad_perm<0x1B>(acc2[i]);  // lane ^ 3
           // column c & 3 = 0: acc1 + p2, 1: acc1 - p2, 2: p2 - acc1, 3: -p2 - acc1
-          const float sa = (c & 2) ? -1.f : 1.f, sb = (c & 1) ? -1.f : 1.f;
+          const R sa = (c & 2) ? R(-1) : R(1), sb = (c & 1) ? R(-1) : R(1);
 #pragma unroll
           for (int i = 0; i < 4; ++i) out[i] = sa * acc1[i] + sb * p2[i];
         }
@@ -414,21 +427,21 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int pl = 2 * q + h;
-          const float sy = out[2 * h], an = out[2 * h + 1];
+          const R sy = out[2 * h], an = out[2 * h + 1];
           // component ri of the northern and southern coefficient
-          float nv = sy + an, sv = sy - an;
+          R nv = sy + an, sv = sy - an;
           if (KIND == INV_SCALAR && a.dlam) {
             // d/dlambda: (x + i y) i m = -m y + i m x, the other component
             // from the partner column (lane ^ 1)
-            const float fm = (float)m;
-            const float np = quad_perm<0xB1>(nv), spn = quad_perm<0xB1>(sv);
+            const R fm = (R)m;
+            const R np = quad_perm<0xB1>(nv), spn = quad_perm<0xB1>(sv);
             nv = ri ? np * fm : -np * fm;
             sv = ri ? spn * fm : -spn * fm;
           }
           if constexpr (FUSED && !ST) {
             // the partner column (lane ^ 1) holds the other component: sp
             // is s.y for ri = 0 and s.x for ri = 1
-            const float sp = quad_perm<0xB1>(sv);  // lane ^ 1
+            const R sp = quad_perm<0xB1>(sv);  // lane ^ 1
             if (fl < FB) {
               R* row = (R*)(fbuf + (pl * FB + fl) * NRP);
               if (m == 0) {

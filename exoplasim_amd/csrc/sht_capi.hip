@@ -53,6 +53,7 @@ static void free_tables(Tables& t) {
   for (int i = 0; i < NTAB; ++i) {
     if (t.inv_bf[i]) (void)hipFree(t.inv_bf[i]);
     if (t.inv_f32[i]) (void)hipFree(t.inv_f32[i]);
+    if (t.inv_f64[i]) (void)hipFree(t.inv_f64[i]);
     if (t.dir_f32[i]) (void)hipFree(t.dir_f32[i]);
   }
   t = Tables();
@@ -104,8 +105,9 @@ static int build_tables(Tables& t, int ntru, const double* sid, const double* gw
     const long nbf = (long)(t.npair / 8) * tab_pb(ntru);
     HIPCK(hipMalloc(&t.inv_bf[tt], sizeof(uint16_t) * nbf));
     HIPCK(hipMalloc(&t.inv_f32[tt], sizeof(float) * t.inv_elems));
-    k_pack_inv<<<(unsigned)((t.inv_elems + 255) / 256), 256>>>(ntru, t.npair, t.kbmax, tt,
-                                                              t.filt64, t.inv_f32[tt]);
+    HIPCK(hipMalloc(&t.inv_f64[tt], sizeof(double) * t.inv_elems));
+    k_pack_inv<<<(unsigned)((t.inv_elems + 255) / 256), 256>>>(
+        ntru, t.npair, t.kbmax, tt, t.filt64, t.inv_f32[tt], t.inv_f64[tt]);
     HIPCK(hipGetLastError());
     const long nth = nbf / 3;
     k_pack_inv_bf<<<(unsigned)((nth + 255) / 256), 256>>>(ntru, t.npair, tt, t.filt64,

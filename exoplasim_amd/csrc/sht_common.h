@@ -113,6 +113,14 @@ template <int CTRL>
 __device__ __forceinline__ float quad_perm(float x) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
 }
+template <int CTRL>
+__device__ __forceinline__ double quad_perm(double x) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xF, 0xF, false);
+  const unsigned hi =
+      (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 
 // a per-lane byte offset that no buffer covers: such a load returns 0
 constexpr int BUF_NONE = 0x7ff00000;
@@ -565,6 +573,7 @@ __device__ __forceinline__ void fft384_wave(C2<F>* row, const C2<F>* tw, int lan
 // ---- split-BF16 helpers ----
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t bf16_trunc_bits(float x) {
   return (uint16_t)(__float_as_uint(x) >> 16);

@@ -14,7 +14,7 @@ __global__ void k_legini(int ntru, int npair, const double* sid, const double* g
 __global__ void k_filtered(int ntru, int npair, const double* tab, const double* skgpsp,
                            const double* skspgp, double* out64, float* out32);
 __global__ void k_pack_inv(int ntru, int npair, int kbmax, int t, const double* filt64,
-                           float* f32);
+                           float* f32, double* f64);
 __global__ void k_pack_inv_bf(int ntru, int npair, int t, const double* filt64, uint16_t* bf);
 __global__ void k_pack_dir(int ntru, int npair, int rb, int t, const double* filt64, float* out);
 __global__ void k_twiddle(int n, double* tw64, float* tw32);
@@ -38,7 +38,8 @@ struct Tables {
   double* skspgp = nullptr;
   uint16_t* inv_bf[NTAB] = {};  // split-BF16 inverse table operands (3 planes)
   float* inv_f32[NTAB] = {};    // FP32-MFMA inverse table operands
-  long inv_elems = 0;           // elements of one inv_f32 table
+  double* inv_f64[NTAB] = {};   // FP64-MFMA inverse table operands (same layout)
+  long inv_elems = 0;           // elements of one inv_f32 (inv_f64) table
   float* dir_f32[NTAB] = {};    // MFMA direct table operands (qc, qe, qm, qq)
   int dir_rb = 0;               // row tiles of 16 per (m, parity)
   double* tw64 = nullptr;     // [nlon] complex

@@ -149,12 +149,18 @@ __device__ void split3_f64(double x, uint16_t& h, uint16_t& m, uint16_t& l) {
   l = (uint16_t)(ul >> 16);
 }
 
-// ---- table operand of the FP32-MFMA inverse: [pb][m][kb][512] with row =
-// 2*pair_local + cls and k = n - m - 32 kb, stored [k%4][row][k/4] per
-// 512-element block so that one lane of v_mfma_f32_16x16x4_f32 reads its 8
-// values contiguously; zero where n > NTRU or the class does not match ----
+// ---- table operands of the FP32- and FP64-MFMA inverse: [pb][m][kb][512]
+// with MFMA row `row` and k = n - m - 32 kb, stored [k%4][row][k/4] per
+// 512-element block so that one lane of v_mfma_f32_16x16x4_f32 (or
+// v_mfma_f64_16x16x4_f64, the same operand layout) reads its 8 values
+// contiguously; zero where n > NTRU or the class does not match.  Both
+// instructions leave D[row][col] in lane 16 q + col, but in item v = row % 4
+// of q = row / 4 (FP32) or item v = row / 4 of q = row % 4 (FP64); the rows
+// are ordered so that item v of lane q holds pair 2q + v/2, class v % 2 in
+// both: FP32 row = 2 pair + class, FP64 row = (row % 4 = pair / 2, row / 4 =
+// 2 (pair % 2) + class) ----
 __global__ void k_pack_inv(int ntru, int npair, int kbmax, int t, const double* filt64,
-                           float* f32) {
+                           float* f32, double* f64) {
   const int ncsp = (ntru + 1) * (ntru + 2) / 2;
   const long tsz = (long)npair * ncsp;
   const int npb = npair / 8;
@@ -166,16 +172,19 @@ __global__ void k_pack_inv(int ntru, int npair, int kbmax, int t, const double* 
   const int kb = (int)((i / 512) % kbmax);
   const int m = (int)((i / (512L * kbmax)) % (ntru + 1));
   const int pb = (int)(i / (512L * kbmax * (ntru + 1)));
-  const int pl = row >> 1, cls = row & 1;
   const int n = m + kb * 32 + k;
-  double v = 0.0;
-  if (n <= ntru) {
+  // the value of (pair pl, class cls) at n
+  auto val = [&](int pl, int cls) -> double {
+    if (n > ntru) return 0.0;
     const bool even = ((n - m) & 1) == 0;
     const int mycls = (even != table_is_q(t)) ? 0 : 1;
-    if (mycls == cls) v = filt64[t * tsz + (long)(pb * 8 + pl) * ncsp + w0_of(ntru, m) + (n - m)];
-  }
+    return mycls == cls ? filt64[t * tsz + (long)(pb * 8 + pl) * ncsp + w0_of(ntru, m) + (n - m)]
+                        : 0.0;
+  };
   const long blk = i - (i % 512);
-  f32[blk + ((k % 4) * 16 + row) * 8 + k / 4] = (float)v;
+  const long o = blk + ((k % 4) * 16 + row) * 8 + k / 4;
+  f32[o] = (float)val(row >> 1, row & 1);
+  f64[o] = val(2 * (row % 4) + (row / 4) / 2, (row / 4) % 2);
 }
 
 // ---- table operand of the split-BF16 inverse (layout in sht_common.h):

@@ -79,6 +79,9 @@ static int inv_tv(const Tables& tb, bool fused, const InvArgs& a, hipStream_t s)
   } else if (MODE == MODE_F32MFMA) {
     A0 = tb.inv_f32[t0];
     A1 = tb.inv_f32[TQU];
+  } else if (MODE == MODE_F64) {
+    A0 = tb.inv_f64[t0];
+    A1 = tb.inv_f64[TQU];
   }
   const void* tv = (MODE == MODE_F64) ? (const void*)tb.filt64 : (const void*)tb.filt32;
   const void* tw = (MODE == MODE_F64) ? (const void*)tb.tw64 : (const void*)tb.tw32;
@@ -128,6 +131,11 @@ static int inv_rt_tv(const Tables& tb, const InvArgs& aq, const InvArgs& aj, con
     tj = tb.inv_f32[TQJ];
     tv0 = tb.inv_f32[TQV];
     tu = tb.inv_f32[TQU];
+  } else if (MODE == MODE_F64) {
+    tq = tb.inv_f64[TQI];
+    tj = tb.inv_f64[TQJ];
+    tv0 = tb.inv_f64[TQV];
+    tu = tb.inv_f64[TQU];
   }
   const void* tv = (MODE == MODE_F64) ? (const void*)tb.filt64 : (const void*)tb.filt32;
   const void* tw = (MODE == MODE_F64) ? (const void*)tb.tw64 : (const void*)tb.tw32;
