@@ -11,7 +11,8 @@
 //   k_fft_rows  the FFTs alone, for the in-place fc2gp/gp2fc entry points.
 //
 // Legendre arithmetic per precision mode:
-//   MODE_F64      FP64 FMA on the vector ALUs (the plain path)
+//   MODE_F64      inverse: v_mfma_f64_16x16x4_f64; direct: FP64 FMA on the
+//                 vector ALUs (the plain path)
 //   MODE_F32      FP32 FMA on the vector ALUs
 //   MODE_F32MFMA  v_mfma_f32_16x16x4_f32, FP32 accumulate
 //   MODE_BF16X3   v_mfma_f32_16x16x32_bf16 on an exact three-way bf16 split
