@@ -277,9 +277,15 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
       uint4 r[2];
       bool fix;
     };
-    // 16-byte table loads per item and term: three bf16 planes, or 8 values
-    // of 4 or 8 bytes
-    constexpr int NA = (MODE == MODE_BF16X3) ? 3 : (8 * AE) / 16;
+    // BF16X3 table planes over lane pairs (two loads per item and term, in
+    // fetch below) from T63 up: there the load instructions bound the
+    // Legendre phase (T63 inverse 5% faster); at T21 and T42 the pair
+    // exchange's VALU work cost more than the load it saves (T42 inverse
+    // 1-3% slower)
+    constexpr bool PAIRLOAD = (MODE == MODE_BF16X3) && T >= 63;
+    // 16-byte table loads per item and term: three bf16 planes (two with
+    // PAIRLOAD), or 8 values of 4 or 8 bytes
+    constexpr int NA = (MODE == MODE_BF16X3) ? (PAIRLOAD ? 2 : 3) : (8 * AE) / 16;
     // Every step fetches, also past the last item (m > T: nothing is loaded,
     // at the offsets of m = T), so that each step issues the same loads and
     // the wait for an item's operands counts only the loads issued before
@@ -299,15 +305,40 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
         // absolute -> relative vorticity at (m=0, n=1, re), the first odd
         // value (subtracted when the item is multiplied)
         br.fix = live && m == 0 && kb == 0 && isvor && q == 2;
+        // PAIRLOAD: rows c and c ^ 1 are the two classes of one latitude
+        // pair, so of the lanes c and c ^ 1 of a quarter exactly one holds
+        // nonzero values.  That lane loads its hi plane (first load) and lo
+        // plane (second); the other lane loads the partner's mid plane in
+        // the first load, which the pair exchanges when the item is
+        // multiplied: two 16-byte loads per term instead of three (a wave's
+        // load costs the same however many of its lanes read).  Else each
+        // lane loads its three planes, the other class's lanes nothing.
         const int aso = (tab_seg(T, m) + 16 * kb) * 2;
-        const int av = (c * rw + jq) * 2;
-        const int av0 = (amine0 && inrow) ? av : BUF_NONE;
+        const int av = (c * rw + jq) * 2;                          // own row, hi plane
+        if constexpr (!PAIRLOAD) {
+          const int av0 = (amine0 && inrow) ? av : BUF_NONE;
 #pragma unroll
-        for (int p = 0; p < 3; ++p) a0[p] = buffer_load_x4(rsA0, av0, aso + p * rw * 32);
+          for (int p = 0; p < 3; ++p) a0[p] = buffer_load_x4(rsA0, av0, aso + p * rw * 32);
+          if (NTERM == 2) {
+            const int av1 = (amine1 && inrow) ? av : BUF_NONE;
+#pragma unroll
+            for (int p = 0; p < 3; ++p) a1[p] = buffer_load_x4(rsA1, av1, aso + p * rw * 32);
+          }
+          return;
+        }
+        const int avp = ((c ^ 1) * rw + jq) * 2 + rw * 32;         // partner's row, mid plane
+        const int avl = av + rw * 64;                              // own row, lo plane
+        {
+          const int v1 = inrow ? (amine0 ? av : avp) : BUF_NONE;
+          const int v2 = (amine0 && inrow) ? avl : BUF_NONE;
+          a0[0] = buffer_load_x4(rsA0, v1, aso);
+          a0[1] = buffer_load_x4(rsA0, v2, aso);
+        }
         if (NTERM == 2) {
-          const int av1 = (amine1 && inrow) ? av : BUF_NONE;
-#pragma unroll
-          for (int p = 0; p < 3; ++p) a1[p] = buffer_load_x4(rsA1, av1, aso + p * rw * 32);
+          const int v1 = inrow ? (amine1 ? av : avp) : BUF_NONE;
+          const int v2 = (amine1 && inrow) ? avl : BUF_NONE;
+          a1[0] = buffer_load_x4(rsA1, v1, aso);
+          a1[1] = buffer_load_x4(rsA1, v2, aso);
         }
       } else {
         // m > T: n > T loads nothing, and the table offset lies past the
@@ -350,12 +381,24 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     // and the large ones), so that consecutive MFMAs do not all wait for the
     // previous result
     using Acc = typename std::conditional<MODE == MODE_F64, f64x4, f32x4>::type;
-    auto mul = [&](const BFrag& br, const BF8* bp, const uint4* a, Acc& acc, Acc& acc2) {
+    auto mul = [&](const BFrag& br, const BF8* bp, const uint4* a, bool amine, Acc& acc,
+                   Acc& acc2) {
       if constexpr (MODE == MODE_BF16X3) {
+        // the lane holding its row's values: hi and lo as loaded, mid from
+        // the partner lane (lane ^ 1); the other lane: zeros
         BF8 ah, am, al;
-        ah.q = a[0];
-        am.q = a[1];
-        al.q = a[2];
+        if constexpr (PAIRLOAD) {
+          const uint4 z = {0u, 0u, 0u, 0u};
+          const uint4 pm = {quad_perm_u<0xB1>(a[0].x), quad_perm_u<0xB1>(a[0].y),
+                            quad_perm_u<0xB1>(a[0].z), quad_perm_u<0xB1>(a[0].w)};
+          ah.q = amine ? a[0] : z;
+          am.q = amine ? pm : z;
+          al.q = a[1];
+        } else {
+          ah.q = a[0];
+          am.q = a[1];
+          al.q = a[NA - 1];
+        }
         acc = mfma_bf16(ah, bp[2], acc);
         acc2 = mfma_bf16(am, bp[0], acc2);
         acc = mfma_bf16(am, bp[1], acc);
@@ -403,8 +446,8 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
       fetch(ahead.m, ahead.kb, into.b, into.a0, into.a1);
       BF8 bp[3];
       split(use.b, bp);
-      mul(use.b, bp, use.a0, acc1, acc1b);
-      if (NTERM == 2) mul(use.b, bp, use.a1, acc2, acc2b);
+      mul(use.b, bp, use.a0, amine0, acc1, acc1b);
+      if (NTERM == 2) mul(use.b, bp, use.a1, amine1, acc2, acc2b);
       // (a step past the wave's last item multiplies the zeros it loaded
       // and stores nothing)
       if (next_item(cur).kb == 0 && cur.m <= T) {

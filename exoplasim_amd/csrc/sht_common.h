@@ -114,6 +114,10 @@ __device__ __forceinline__ float quad_perm(float x) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
 }
 template <int CTRL>
+__device__ __forceinline__ unsigned quad_perm_u(unsigned x) {
+  return (unsigned)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
 __device__ __forceinline__ double quad_perm(double x) {
   const unsigned long long u = (unsigned long long)__double_as_longlong(x);
   const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xF, 0xF, false);
