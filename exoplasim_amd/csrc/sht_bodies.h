@@ -849,6 +849,10 @@ __device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* 
   else
     outp = (R*)scratch + ((long)pb * a.nmem + mem) * a.nfield * S::NRSP;
   const bool qtab = table_is_q(a.table);
+  // FP64: the table [t][pair][w], one 8-byte load per pair; FP32: [t][pair /
+  // 4][w][4], one 16-byte load per 4 pairs (a wave's loads cost the same
+  // whatever their width, so the wider load reads 4x the bytes per load)
+  static_assert(sizeof(R) == 8 || PB % 4 == 0, "pair blocks of whole groups of 4");
   const R* tv = (const R*)tabv + (long)a.table * S::NPAIR * S::NCSP + (long)pb * PB * S::NCSP;
   const __amdgpu_buffer_rsrc_t trs = buffer_rsrc(tv, PB * S::NCSP * (int)sizeof(R));
   for (int w = tid; w < S::NCSP; w += NT) {
@@ -859,11 +863,24 @@ __device__ __forceinline__ void fwd_body(const FwdArgs& a, int blk, const void* 
     R acc[NCOL];
 #pragma unroll
     for (int c = 0; c < NCOL; ++c) acc[c] = R(0);
+    if constexpr (sizeof(R) == 8) {
 #pragma unroll 8
-    for (int pl = 0; pl < PB; ++pl) {
-      const R p = buffer_load<R>(trs, w * (int)sizeof(R), pl * S::NCSP * (int)sizeof(R));
+      for (int pl = 0; pl < PB; ++pl) {
+        const R p = buffer_load<R>(trs, w * (int)sizeof(R), pl * S::NCSP * (int)sizeof(R));
 #pragma unroll
-      for (int c = 0; c < NCOL; ++c) acc[c] += p * fr[pl * NCOL + c];
+        for (int c = 0; c < NCOL; ++c) acc[c] += p * fr[pl * NCOL + c];
+      }
+    } else {
+#pragma unroll 2
+      for (int pq = 0; pq < PB / 4; ++pq) {
+        const uint4 p4 = buffer_load_x4(trs, w * 16, pq * S::NCSP * 16);
+        const R p[4] = {__uint_as_float(p4.x), __uint_as_float(p4.y), __uint_as_float(p4.z),
+                        __uint_as_float(p4.w)};
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int c = 0; c < NCOL; ++c) acc[c] += p[u] * fr[(4 * pq + u) * NCOL + c];
+      }
     }
 #pragma unroll
     for (int c = 0; c < NCOL; ++c) {

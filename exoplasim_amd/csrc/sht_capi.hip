@@ -47,7 +47,7 @@ extern "C" int exo_device_count(void) {
 }
 
 static void free_tables(Tables& t) {
-  void* ps[] = {t.sid, t.gwd, t.tab, t.filt64, t.filt32, t.skgpsp, t.skspgp, t.tw64, t.tw32};
+  void* ps[] = {t.sid, t.gwd, t.tab, t.filt64, t.filt32, t.fwd4, t.skgpsp, t.skspgp, t.tw64, t.tw32};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   for (int i = 0; i < NTAB; ++i) {
@@ -76,6 +76,7 @@ static int build_tables(Tables& t, int ntru, const double* sid, const double* gw
   HIPCK(hipMalloc(&t.tab, sizeof(double) * NTAB * tsz));
   HIPCK(hipMalloc(&t.filt64, sizeof(double) * NTAB * tsz));
   HIPCK(hipMalloc(&t.filt32, sizeof(float) * NTAB * tsz));
+  HIPCK(hipMalloc(&t.fwd4, sizeof(float) * NTAB * tsz));
   HIPCK(hipMalloc(&t.skgpsp, sizeof(double) * (ntru + 1)));
   HIPCK(hipMalloc(&t.skspgp, sizeof(double) * (ntru + 1)));
   HIPCK(hipMalloc(&t.tw64, sizeof(double) * 2 * t.nlon));
@@ -98,6 +99,8 @@ static int build_tables(Tables& t, int ntru, const double* sid, const double* gw
   const long nf = NTAB * tsz;
   k_filtered<<<(unsigned)((nf + 255) / 256), 256>>>(ntru, t.npair, t.tab, t.skgpsp, t.skspgp,
                                                     t.filt64, t.filt32);
+  HIPCK(hipGetLastError());
+  k_pack_fwd4<<<(unsigned)((nf + 255) / 256), 256>>>(t.npair, t.ncsp, t.filt32, t.fwd4);
   HIPCK(hipGetLastError());
   t.inv_elems = (long)(t.npair / 8) * (ntru + 1) * t.kbmax * 512;
   const int inv_tabs[] = {TQI, TQJ, TQU, TQV};

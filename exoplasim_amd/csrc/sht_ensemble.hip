@@ -363,7 +363,7 @@ static int ens_prepare(exo_sht_ens_s* e) {
     f.a.nfield = nfl;
     f.a.table = TQC;
     f.a.var = e->fwd_var[k];
-    f.tv = tv;
+    f.tv = tb.fwd4;  // the direct sums' FP32 tables, 4 pairs side by side
     f.tw = tw;
     f.ntru = tb.ntru;
     f.var = e->fwd_var[k];

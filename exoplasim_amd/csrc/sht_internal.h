@@ -17,6 +17,7 @@ __global__ void k_pack_inv(int ntru, int npair, int kbmax, int t, const double* 
                            float* f32, double* f64);
 __global__ void k_pack_inv_bf(int ntru, int npair, int t, const double* filt64, uint16_t* bf);
 __global__ void k_pack_dir(int ntru, int npair, int rb, int t, const double* filt64, float* out);
+__global__ void k_pack_fwd4(int npair, int ncsp, const float* filt32, float* fwd4);
 __global__ void k_twiddle(int n, double* tw64, float* tw32);
 
 // Which inverse transform a launch performs.
@@ -34,6 +35,7 @@ struct Tables {
   double* tab = nullptr;      // [8][npair][ncsp] unfiltered
   double* filt64 = nullptr;   // [8][npair][ncsp] filter folded in
   float* filt32 = nullptr;
+  float* fwd4 = nullptr;      // [8][npair/4][ncsp][4]: filt32 with 4 pairs side by side
   double* skgpsp = nullptr;   // [ntru+1]
   double* skspgp = nullptr;
   uint16_t* inv_bf[NTAB] = {};  // split-BF16 inverse table operands (3 planes)

@@ -241,6 +241,22 @@ __global__ void k_pack_dir(int ntru, int npair, int rb, int t, const double* fil
                        : 0.0f;
 }
 
+// ---- the FP32 tables of the fused direct transform (k_fwd): 4 latitude
+// pairs side by side, [t][pair / 4][w][pair % 4], so that a lane summing
+// coefficient w reads 4 pairs in one 16-byte load and the lanes of a wave
+// (consecutive w) read one contiguous run ----
+__global__ void k_pack_fwd4(int npair, int ncsp, const float* filt32, float* fwd4) {
+  const long total = (long)NTAB * npair * ncsp;
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int u = (int)(i % 4);
+  const int w = (int)((i / 4) % ncsp);
+  const long tg = i / (4L * ncsp);  // t * (npair / 4) + pair group
+  const int g = (int)(tg % (npair / 4));
+  const int t = (int)(tg / (npair / 4));
+  fwd4[i] = filt32[((long)t * npair + 4 * g + u) * ncsp + w];
+}
+
 // twiddles e^{+2 pi i k/N} in FP64 and FP32
 __global__ void k_twiddle(int n, double* tw64, float* tw32) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;

@@ -169,7 +169,8 @@ static int fwd_tv(const Tables& tb, bool fused, const FwdArgs& a, void* scratch,
   if (nblk <= 0) return 0;
   if (C::NPB > 1 && (scratch == nullptr || sbytes < fwd_scratch_t<T, MODE>(a.nmem, a.nfield)))
     return -4;
-  const void* tv = (MODE == MODE_F64) ? (const void*)tb.filt64 : (const void*)tb.filt32;
+  // the direct sums read FP64 tables [t][pair][w], FP32 ones grouped by 4 pairs
+  const void* tv = (MODE == MODE_F64) ? (const void*)tb.filt64 : (const void*)tb.fwd4;
   const void* tw = (MODE == MODE_F64) ? (const void*)tb.tw64 : (const void*)tb.tw32;
   dim3 grid((unsigned)nblk), block(C::NT);
   if (fused)
