@@ -29,7 +29,9 @@ struct InvCfg {
   // compact Fourier coefficients (m <= T, 16 bytes per row and m: 128 KB for
   // 8 pairs x 8 fields, all 16 MFMA columns) and each wave then transforms
   // its rows one at a time in a row buffer of its own.  Full FFT rows of 8
-  // fields (202 KB) would not fit a CU's 160 KB of LDS.
+  // fields (202 KB) would not fit a CU's 160 KB of LDS.  (The same layout at
+  // T63, 78 KB and two blocks per CU, made the 32-member inverse 4% faster
+  // but the round trip no faster and the ensemble 1.5% slower.)
   static constexpr bool STAGED = (T == 127) && (MODE != MODE_F64);
   static constexpr int FB0 =
       (T == 127) ? (MODE == MODE_F64 ? 2 : 8) : (MODE == MODE_F64 ? 4 : 8);
@@ -616,7 +618,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     // a time in its own row buffer; the first FFT pass builds
     // z = fc_N + i fc_S with its Hermitian extension from the compact
     // coefficients as it reads them.  Re -> north row, Im -> south row ----
-    static_assert(N == 384, "the staged inverse runs the 384-point wave FFT");
+    static_assert(N == 192 || N == 384, "the staged inverse runs the wave FFT of 192 or 384");
     const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     C2<R>* wrow = fbuf + wv * NRP;
     typedef R R4 __attribute__((ext_vector_type(4)));
@@ -625,19 +627,20 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
       const int d = dst_of(fl);
       if (d < 0) continue;
       const NS2<R>* e = fcl + f * NM;
-      // input k = lane + 64 r: r = 0, 1 the stored m = k (z_0 = (n.x, s.x),
-      // the imaginary parts of m = 0 ignored as fc2gp does); r = 2, 3 and
-      // k = 256 zero (above the truncation); else z_{N-m}, m = N - k
+      // input k = lane + 64 r: k <= T the stored m = k (z_0 = (n.x, s.x),
+      // the imaginary parts of m = 0 ignored as fc2gp does); T < k < N - T
+      // zero (above the truncation; whole passes r of it known at compile
+      // time); else z_{N-m}, m = N - k
       auto load = [&](int r, int k) -> C2<R> {
-        if (r < 2) {
+        if (64 * r + 63 <= T || (64 * r <= T && k <= T)) {
           const NS2<R> v = e[k];
           return (k == 0) ? C2<R>{v.n.x, v.s.x} : C2<R>{v.n.x - v.s.y, v.n.y + v.s.x};
         }
-        if (r < 4 || k < N - T) return C2<R>{R(0), R(0)};
+        if (64 * r + 63 < N - T || k < N - T) return C2<R>{R(0), R(0)};
         const NS2<R> v = e[N - k];
         return C2<R>{v.n.x + v.s.y, v.s.x - v.n.y};
       };
-      fft384_wave<+1, R>(wrow, tws, lane, load);
+      fft_wave<N, +1, R>(wrow, tws, lane, load);
       // whole rows in runs of 4 values per lane, non-temporal (as below)
       const int lat = pb * LP + pl;
       R* o = outm + (long)d * NLAT * N;

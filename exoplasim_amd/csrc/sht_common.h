@@ -525,51 +525,55 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// FFT of one row of 384 complex values by one wave (lane = 0..63), with
-// passes of radix 6, 8 and 8 (Stockham autosort; the first pass has one
-// butterfly per lane).  The first pass reads input k through load(k); the
-// transform is left in `row` (384 + 12 complex, padded as bix).  tw[k] =
-// e^{S 2 pi i k / 384}.  Only the calling wave touches `row`.
-template <int S, typename F, typename Load>
-__device__ __forceinline__ void fft384_wave(C2<F>* row, const C2<F>* tw, int lane, Load& load) {
-  {  // radix 6, stride 1: butterfly j = lane takes k = j + 64 r
-    C2<F> v[6];
+// FFT of one row of N = 64 R0 complex values (R0 = 3 or 6: 192 or 384) by
+// one wave (lane = 0..63), with passes of radix R0, 8 and 8 (Stockham
+// autosort; the first pass has one butterfly per lane).  The first pass
+// reads input k through load(r, k), k = lane + 64 r; the transform is left
+// in `row` (N + N/32 complex, padded as bix).  tw[k] = e^{S 2 pi i k / N}.
+// Only the calling wave touches `row`.
+template <int N, int S, typename F, typename Load>
+__device__ __forceinline__ void fft_wave(C2<F>* row, const C2<F>* tw, int lane, Load& load) {
+  constexpr int R0 = N / 64;  // first radix
+  constexpr int NB = N / 8;   // butterflies of the radix-8 passes (lanes 0 .. NB-1)
+  static_assert(N == 64 * R0 && (R0 == 3 || R0 == 6), "rows of 192 or 384");
+  {  // radix R0, stride 1: butterfly j = lane takes k = j + 64 r
+    C2<F> v[R0];
 #pragma unroll
-    for (int r = 0; r < 6; ++r) v[r] = load(r, lane + 64 * r);
-    Dft<6, S, F>::run(v);
+    for (int r = 0; r < R0; ++r) v[r] = load(r, lane + 64 * r);
+    Dft<R0, S, F>::run(v);
 #pragma unroll
-    for (int r = 0; r < 6; ++r) row[bix(6 * lane + r)] = v[r];
+    for (int r = 0; r < R0; ++r) row[bix(R0 * lane + r)] = v[r];
   }
   wave_lds_sync();
-  // radix 8, stride 6, then radix 8, stride 48: butterflies j < 48 take
-  // k = j + 48 r; in place, all reads of the wave before its writes
+  // radix 8, stride R0, then radix 8, stride 8 R0: butterflies j < NB take
+  // k = j + NB r; in place, all reads of the wave before its writes
   C2<F> v[8];
-  if (lane < 48) {
+  if (lane < NB) {
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = row[bix(lane + 48 * r)];
+    for (int r = 0; r < 8; ++r) v[r] = row[bix(lane + NB * r)];
   }
   wave_lds_sync();
-  if (lane < 48) {
-    const int jm = lane % 6;
+  if (lane < NB) {
+    const int jm = lane % R0;
 #pragma unroll
     for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], tw[jm * r * 8]);
     Dft<8, S, F>::run(v);
-    const int o = (lane / 6) * 48 + jm;
+    const int o = (lane / R0) * (8 * R0) + jm;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) row[bix(o + 6 * r)] = v[r];
+    for (int r = 0; r < 8; ++r) row[bix(o + R0 * r)] = v[r];
   }
   wave_lds_sync();
-  if (lane < 48) {
+  if (lane < NB) {
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = row[bix(lane + 48 * r)];
+    for (int r = 0; r < 8; ++r) v[r] = row[bix(lane + NB * r)];
   }
   wave_lds_sync();
-  if (lane < 48) {
+  if (lane < NB) {
 #pragma unroll
     for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], tw[lane * r]);
     Dft<8, S, F>::run(v);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) row[bix(lane + 48 * r)] = v[r];
+    for (int r = 0; r < 8; ++r) row[bix(lane + NB * r)] = v[r];
   }
   wave_lds_sync();
 }
