@@ -142,9 +142,12 @@ __global__ __launch_bounds__(512) void k_dirx_mf(DirxArgs a, DirxSlots sl) {
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nlg = (a.nlev + LG - 1) / LG;
-  const int mr = blockIdx.x % C::NMR;
-  const int lg = (blockIdx.x / C::NMR) % nlg;
-  const int mem = blockIdx.x / (C::NMR * nlg);
+  // neighbouring wavenumber ranges stage the two halves of the same 128-byte
+  // lines of a Fourier row: renumbered so that they run on one XCD's L2
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int mr = b % C::NMR;
+  const int lg = (b / C::NMR) % nlg;
+  const int mem = b / (C::NMR * nlg);
   if (mem >= a.nmem) return;
   const int m0 = mr * MR, lev0 = lg * LG;
   const int nj = (S::NM - m0 < MR ? S::NM - m0 : MR) * 2;  // reals below the truncation
