@@ -242,6 +242,9 @@ struct exo_sht_ens_s {
   exo_sht_plan_t plan[ENS_MAXGROUP] = {};  // one per group (own tables)
   float* pk[ENS_MAXGROUP] = {};            // BF16X3: parity-ordered spectral input per group
   int inv_var[ENS_MAXGROUP] = {}, fwd_var[ENS_MAXGROUP] = {};
+  // tile variants (inverse, inverse_uv, direct) of each group's batched
+  // round trip; -1: those of the persistent kernels
+  int bvar[ENS_MAXGROUP][3];
   // device state
   EnsInvJob* dinv = nullptr;
   EnsFwdJob* dfwd = nullptr;
@@ -469,12 +472,20 @@ static int ens_pack(exo_sht_ens_s* e, hipStream_t s) {
   return EXO_OK;
 }
 
+// the tile variants of group k's per-truncation kernels: its own batched
+// variants once the autotuner has chosen them, else the persistent kernels'
+static int ens_plan_tiles(exo_sht_ens_s* e, int k) {
+  if (e->bvar[k][0] >= 0)
+    return exo_sht_set_tiles(e->plan[k], e->mode, e->bvar[k][0], e->bvar[k][1], e->bvar[k][2]);
+  return exo_sht_set_tiles(e->plan[k], e->mode, plan_var(e->inv_var[k]), plan_var(e->inv_var[k]),
+                           plan_var(e->fwd_var[k]));
+}
+
 // one batched round trip per group, with the group's tile variants
 static int ens_groups(exo_sht_ens_s* e, hipStream_t s) {
   for (int k = 0; k < e->ngroup; ++k) {
     const exo_sht_ens_group& g = e->g[k];
-    int rc = exo_sht_set_tiles(e->plan[k], e->mode, plan_var(e->inv_var[k]),
-                               plan_var(e->inv_var[k]), plan_var(e->fwd_var[k]));
+    int rc = ens_plan_tiles(e, k);
     if (rc == EXO_OK)
       rc = exo_sht_roundtrip(e->plan[k], e->mode, g.nmem, g.nlev, g.sp_in, g.gp, g.sp_out, s);
     if (rc != EXO_OK) return rc;
@@ -554,6 +565,7 @@ extern "C" int exo_sht_ens_create(exo_sht_ens_t* ens, int mode, int ngroup,
   exo_sht_ens_s* e = new exo_sht_ens_s();
   e->mode = mode;
   e->ngroup = ngroup;
+  for (int k = 0; k < ENS_MAXGROUP; ++k) e->bvar[k][0] = e->bvar[k][1] = e->bvar[k][2] = -1;
   int rc = EXO_OK;
   for (int k = 0; k < ngroup && rc == EXO_OK; ++k) {
     e->g[k] = groups[k];
@@ -626,9 +638,7 @@ extern "C" int exo_sht_ens_run_members(exo_sht_ens_t e, void* stream) {
     const exo_sht_ens_group& g = e->g[k];
     int dims[4];
     int rc = exo_sht_plan_dims(e->plan[k], dims);
-    if (rc == EXO_OK)
-      rc = exo_sht_set_tiles(e->plan[k], e->mode, plan_var(e->inv_var[k]), plan_var(e->inv_var[k]),
-                               plan_var(e->fwd_var[k]));
+    if (rc == EXO_OK) rc = ens_plan_tiles(e, k);
     if (rc != EXO_OK) return rc;
     const int nfl = 3 * g.nlev + 1, ngp = nfl + 1 + 2 * g.nlev;
     const long sps = (long)nfl * 2 * dims[3], gps = (long)ngp * dims[1] * dims[2];
@@ -668,6 +678,7 @@ extern "C" int exo_sht_ens_set_tiles(exo_sht_ens_t e, const int* vars) {
   for (int k = 0; k < e->ngroup; ++k) {
     e->inv_var[k] = vars[2 * k];
     e->fwd_var[k] = vars[2 * k + 1];
+    e->bvar[k][0] = e->bvar[k][1] = e->bvar[k][2] = -1;  // both schedules run these
   }
   int rc = ens_prepare(e);
   if (rc == EXO_OK) rc = ens_capture(e);
@@ -676,9 +687,10 @@ extern "C" int exo_sht_ens_set_tiles(exo_sht_ens_t e, const int* vars) {
 
 // Times each combination of tile variants per truncation (the groups of one
 // truncation share a variant: 0 or 1 with 256-thread blocks, 2 or 3 with
-// 512-thread blocks) for the inverse and the direct kernel, keeps the
-// fastest, then times the two schedules with those variants and captures
-// the faster in the graph.
+// 512-thread blocks) for the inverse and the direct kernel of the
+// persistent schedule, keeps the fastest, lets each group's plan choose the
+// variants of its own batched round trip (exo_sht_autotune), then times the
+// two schedules and captures the faster in the graph.
 extern "C" int exo_sht_ens_autotune(exo_sht_ens_t e, void* stream, int* vars) {
   if (!e) return EXO_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
@@ -733,6 +745,8 @@ extern "C" int exo_sht_ens_autotune(exo_sht_ens_t e, void* stream, int* vars) {
     e->fwd_var[k] = (pick[1] >> 3) * 2 + bf;
   }
   if (rc == EXO_OK) rc = ens_prepare(e);
+  for (int k = 0; k < e->ngroup && rc == EXO_OK; ++k)
+    rc = exo_sht_autotune(e->plan[k], e->mode, e->g[k].nmem, e->g[k].nlev, stream, e->bvar[k]);
   // the whole round trip in each schedule (the first pass of each sizes the
   // plans' scratch and is not timed)
   float tsched[2] = {1e30f, 1e30f};

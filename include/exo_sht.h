@@ -277,8 +277,10 @@ int exo_sht_ens_autotune(exo_sht_ens_t ens, void *stream, int *vars);
  * or one batched exo_sht_roundtrip per group (EXO_ENS_BATCHED, the kernels
  * of exo_sht_ens_run_groups).  set_schedule captures the graph again (for
  * EXO_ENS_BATCHED after one direct pass, which writes the outputs);
- * autotune, after choosing the tile variants, times both schedules and
- * keeps the faster. */
+ * autotune, after choosing the persistent kernels' tile variants, lets each
+ * group's plan choose those of its batched round trip (exo_sht_autotune),
+ * times both schedules and keeps the faster.  set_tiles sets the variants
+ * of both schedules. */
 #define EXO_ENS_PERSISTENT 0
 #define EXO_ENS_BATCHED 1
 int exo_sht_ens_get_schedule(exo_sht_ens_t ens, int *sched);
