@@ -203,21 +203,26 @@ static int fft_t(const void* tw, int dir, R* a, long lot, int nmem, long mstride
 
 // ---------------------------------------------------------------------------
 // parity-ordered spectral operand of the split-BF16 inverse (layout in
-// sht_common.h): one thread per 8 consecutive values of one (member, field,
-// m, re/im, parity) row, written as two 16-byte chunks, so consecutive
-// threads fill a row contiguously
+// sht_common.h): one thread per 4 consecutive coefficients n - m = 4g ..
+// 4g + 3 of one (member, field, m), read as two contiguous 16-byte loads
+// (consecutive threads read consecutive runs) and written as 8 bytes, the
+// values j = 2g, 2g + 1, into each of the m's four rows (re/im x parity),
+// so a wave writes 512 contiguous bytes of each row.  Coefficients past the
+// truncation are zeros.  (Each thread reading 8 values of one row at a
+// 16-byte stride put a wave's loads on 64 cache lines each.)
 // ---------------------------------------------------------------------------
 template <int T>
 __global__ __launch_bounds__(256) void k_pack_sp(const float* sp, long sp_stride, int f0, int nf,
                                                  int nmem, float* out) {
   using S = TS<T>;
-  constexpr int NC = pk_c16(T + 1) / 2;  // 8-value chunks per field (4 rows per m)
+  constexpr int NC = pk_c16(T + 1) / 2;  // threads per field: sum over m of rw(m) / 2
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i >= (long)nmem * nf * NC) return;
   const int u = (int)(i % NC);
   const long fm = i / NC;
   const int field = (int)(fm % nf), mem = (int)(fm / nf);
-  // the largest m whose segment starts at or before chunk u
+  // the largest m whose segment starts at or before thread u (segment of m:
+  // threads pk_seg(m) / 8 .. + rw(m) / 2)
   int lo = 0, hi = T;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -227,19 +232,27 @@ __global__ __launch_bounds__(256) void k_pack_sp(const float* sp, long sp_stride
       hi = mid - 1;
   }
   const int m = lo, rw = pk_rw(T, m);
-  const int r = u - pk_seg(T, m) / 8;  // chunk within the segment's 4 rows
-  const int row = r / (rw / 8), j0 = (r - row * (rw / 8)) * 8;
-  const int ri = row >> 1, h = row & 1;
-  const float* src = sp + mem * sp_stride + (long)(f0 + field) * S::NRSP + 2 * w0_of(T, m) + ri;
-  float x[8];
+  const int g = u - pk_seg(T, m) / 8;  // coefficients n - m = 4g .. 4g + 3
+  // the member's fields as one buffer: a run past its end reads zeros, one
+  // past the truncation (into the next m or field) is zeroed below
+  const float* base = sp + mem * sp_stride;
+  const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(base, (int)((f0 + nf) * (long)S::NRSP * 4));
+  const int off = (int)(((long)(f0 + field) * S::NRSP + 2 * (w0_of(T, m) + 4 * g)) * 4);
+  const uint4 a0 = buffer_load_x4(rs, off, 0), a1 = buffer_load_x4(rs, off, 16);
+  float x[8] = {__uint_as_float(a0.x), __uint_as_float(a0.y), __uint_as_float(a0.z),
+                __uint_as_float(a0.w), __uint_as_float(a1.x), __uint_as_float(a1.y),
+                __uint_as_float(a1.z), __uint_as_float(a1.w)};
 #pragma unroll
-  for (int jj = 0; jj < 8; ++jj) {
-    const int dn = 2 * (j0 + jj) + h;  // n - m
-    x[jj] = (m + dn <= T) ? src[2 * dn] : 0.f;
-  }
-  float* dst = out + fm * pk_field(T) + pk_seg(T, m) + row * rw + j0;
-  *(float4*)dst = make_float4(x[0], x[1], x[2], x[3]);
-  *(float4*)(dst + 4) = make_float4(x[4], x[5], x[6], x[7]);
+  for (int q = 0; q < 4; ++q)
+    if (m + 4 * g + q > T) x[2 * q] = x[2 * q + 1] = 0.f;
+  // rows (re/im ri, parity h) = 2 ri + h; values j = 2g (n - m = 4g + h) and
+  // 2g + 1 (4g + 2 + h)
+  float* dst = out + fm * pk_field(T) + pk_seg(T, m) + 2 * g;
+#pragma unroll
+  for (int ri = 0; ri < 2; ++ri)
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+      *(float2*)(dst + (2 * ri + h) * rw) = make_float2(x[2 * h + ri], x[2 * (2 + h) + ri]);
 }
 
 template <int T>
