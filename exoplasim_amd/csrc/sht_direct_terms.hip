@@ -135,10 +135,12 @@ __global__ __launch_bounds__(512) void k_dirx_mf(DirxArgs a, DirxSlots sl) {
   constexpr int NKS = KC / 4;  // MFMA K steps (4 pairs each) per staged step
   constexpr int NJ = 2 * MR;   // reals of one staged row segment
   static_assert(NPAIR % KC == 0, "pairs per step");
-  // staged values per step: (input, level, pair, real) with the real fastest
-  constexpr int NITEM = NIN * LG * KC * NJ;
+  // staged values per step: (input, level, pair, 4 reals) with the reals
+  // fastest, one 16-byte load per item
+  constexpr int NJ4 = NJ / 4;
+  constexpr int NITEM = NIN * LG * KC * NJ4;
   constexpr int MAXIT = (NITEM + NT - 1) / NT;
-  __shared__ float X[NIN * 2 * KC * KS];  // [input][class][pair][level][m][re/im]
+  __shared__ __align__(16) float X[NIN * 2 * KC * KS];  // [input][class][pair][level][m][re/im]
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nlg = (a.nlev + LG - 1) / LG;
@@ -152,20 +154,34 @@ __global__ __launch_bounds__(512) void k_dirx_mf(DirxArgs a, DirxSlots sl) {
   const int m0 = mr * MR, lev0 = lg * LG;
   const int nj = (S::NM - m0 < MR ? S::NM - m0 : MR) * 2;  // reals below the truncation
 
-  float rn[MAXIT], rs[MAXIT];
+  // each input's member as one buffer (2 m0 + NJ <= N: the 16-byte runs
+  // stay inside their rows; reals past the truncation are zeroed)
+  __amdgpu_buffer_rsrc_t rsi[NIN];
+#pragma unroll
+  for (int i = 0; i < NIN; ++i)
+    rsi[i] = buffer_rsrc((const float*)a.in[i] + mem * a.in_stride, a.nlev * NLAT * N * 4);
+  static_assert(2 * ((TS<T>::NM - 1) / MR * MR) + NJ <= N, "staged runs inside a row");
+  float rn[MAXIT][4], rs[MAXIT][4];
   auto load = [&](int c) {
 #pragma unroll
     for (int it = 0; it < MAXIT; ++it) {
       const int e = tid + NT * it;
-      const int j = e % NJ, k = (e / NJ) % KC, l = (e / (NJ * KC)) % LG, i = e / (NJ * KC * LG);
-      rn[it] = 0.f;
-      rs[it] = 0.f;
-      if (e < NITEM && lev0 + l < a.nlev && j < nj) {
-        const float* src = (const float*)a.in[i] + mem * a.in_stride +
-                           (long)(lev0 + l) * NLAT * N + 2 * m0 + j;
+      const int j4 = e % NJ4, k = (e / NJ4) % KC, l = (e / (NJ4 * KC)) % LG,
+                i = e / (NJ4 * KC * LG);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) rn[it][q] = rs[it][q] = 0.f;
+      if (e < NITEM && lev0 + l < a.nlev && 4 * j4 < nj) {
         const int pair = c * KC + k;
-        rn[it] = src[(long)pair * N];
-        rs[it] = src[(long)(NLAT - 1 - pair) * N];
+        const int off = ((lev0 + l) * NLAT * N + 2 * m0 + 4 * j4) * 4;
+        const uint4 vn = buffer_load_x4(rsi[i], off + pair * N * 4, 0);
+        const uint4 vs = buffer_load_x4(rsi[i], off + (NLAT - 1 - pair) * N * 4, 0);
+        const unsigned un[4] = {vn.x, vn.y, vn.z, vn.w}, us[4] = {vs.x, vs.y, vs.z, vs.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (4 * j4 + q < nj) {
+            rn[it][q] = __uint_as_float(un[q]);
+            rs[it][q] = __uint_as_float(us[q]);
+          }
       }
     }
   };
@@ -175,10 +191,15 @@ __global__ __launch_bounds__(512) void k_dirx_mf(DirxArgs a, DirxSlots sl) {
     for (int it = 0; it < MAXIT; ++it) {
       const int e = tid + NT * it;
       if (e < NITEM) {
-        const int j = e % NJ, k = (e / NJ) % KC, l = (e / (NJ * KC)) % LG, i = e / (NJ * KC * LG);
-        const int o = l * NJ + j;
-        X[((i * 2 + 0) * KC + k) * KS + o] = rn[it] + rs[it];
-        X[((i * 2 + 1) * KC + k) * KS + o] = rn[it] - rs[it];
+        const int j4 = e % NJ4, k = (e / NJ4) % KC, l = (e / (NJ4 * KC)) % LG,
+                  i = e / (NJ4 * KC * LG);
+        const int o = l * NJ + 4 * j4;  // even, and KS is even: 8-byte aligned
+        float2* xe = (float2*)&X[((i * 2 + 0) * KC + k) * KS + o];
+        float2* xo = (float2*)&X[((i * 2 + 1) * KC + k) * KS + o];
+        xe[0] = make_float2(rn[it][0] + rs[it][0], rn[it][1] + rs[it][1]);
+        xe[1] = make_float2(rn[it][2] + rs[it][2], rn[it][3] + rs[it][3]);
+        xo[0] = make_float2(rn[it][0] - rs[it][0], rn[it][1] - rs[it][1]);
+        xo[1] = make_float2(rn[it][2] - rs[it][2], rn[it][3] - rs[it][3]);
       }
     }
   };
@@ -318,6 +339,8 @@ static int dirx_mf_t(const Tables& tb, const DirxArgs& a, hipStream_t s) {
     sl.ic[k][tm.oc] = tm.ic;
   }
   if (sl.nslot != a.nin) return -2;  // one table per input
+  // a member's input levels are addressed as one buffer of 32-bit byte offsets
+  if ((long)a.nlev * TS<T>::NLAT * TS<T>::NLON * 4 > 0x7fffffffL) return -2;
   const long nlg = (a.nlev + 7) / 8;
   auto launch = [&](auto kern, int nmr) {
     kern<<<(unsigned)((long)a.nmem * nlg * nmr), 512, 0, s>>>(a, sl);
