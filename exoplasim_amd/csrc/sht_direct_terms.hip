@@ -110,7 +110,7 @@ struct DirxMfCfg {
   static constexpr int NT = 512, NW = NT / 64;
   static constexpr int MPW = (MR + NW - 1) / NW;                     // wavenumbers per wave
   static constexpr int LG = 8;                                       // levels per block
-  static constexpr int KC = (NIN == 1) ? 16 : (T == 127) ? ((NIN == 2) ? 8 : 4) : (NIN == 3) ? 8 : (T == 63) ? 16 : 8;  // pairs per step
+  static constexpr int KC = (NIN == 1) ? ((T == 127) ? 8 : 16) : (T == 127) ? ((NIN == 2) ? 8 : 4) : (NIN == 3) ? 8 : (T == 63) ? 16 : 8;  // pairs per step
   static constexpr int KS = LG * 2 * MR + 2;  // floats per (input, class, pair), padded
 };
 
