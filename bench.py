@@ -41,8 +41,8 @@ HBM_PEAK_GBS = 8000.0
 # script, each scaled by the counter's reading for a 256 MiB device copy
 # (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); the runs and the command are in
 # profiles/t42_l20_hbm_traffic.md.  Keyed by (mode, call, members).
-MEASURED_TRAFFIC = {("bf16x3", "inverse_all", 128): 659.7e6,
-                    ("bf16x3", "direct_qc", 128): 316.3e6}
+MEASURED_TRAFFIC = {("bf16x3", "inverse_all", 128): 658.9e6,
+                    ("bf16x3", "direct_qc", 128): 316.4e6}
 
 
 def parse():
