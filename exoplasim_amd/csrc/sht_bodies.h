@@ -19,7 +19,8 @@ using RealT = typename std::conditional<MODE == MODE_F64, double, float>::type;
 // V selects the tile variant the autotuner times: V = 1 halves the fields
 // per block (half the LDS, twice the blocks in flight); V = 2 keeps the
 // fields and runs the block with 512 threads (8 waves: twice the waves in
-// flight per CU for the same LDS, FFT batches of twice the rows); V = 3
+// flight per CU for the same LDS, FFT batches of twice the rows; the staged
+// T127 block 1024 threads); V = 3
 // halves the fields and runs 512 threads (the persistent ensemble kernels,
 // whose LDS is sized for their largest truncation).
 template <int T, int MODE, int V = 0>
@@ -36,7 +37,10 @@ struct InvCfg {
   static constexpr int FB0 =
       (T == 127) ? (MODE == MODE_F64 ? 2 : 8) : (MODE == MODE_F64 ? 4 : 8);
   static constexpr int FB = (V == 1 || V == 3) ? ((FB0 >> 1) < 2 ? 2 : (FB0 >> 1)) : FB0;
-  static constexpr int NT = (V == 2 || V == 3) ? 512 : 256;  // threads per block
+  // threads per block; the staged V = 2 block runs 16 waves in the Legendre
+  // phase, of which 8 then run the FFT rows (one row buffer each)
+  static constexpr int NT = (V == 2 || V == 3) ? ((STAGED && V == 2) ? 1024 : 512) : 256;
+  static constexpr int NWF = (NT / 64 < 8) ? NT / 64 : 8;  // staged: FFT waves
   // MFMA work items in flight per wave, one table (D1) or two (D2; dv2uv
   // carries two tables per item, and one item ahead keeps its registers
   // within four waves per SIMD)
@@ -92,7 +96,7 @@ struct InvLds {
   static constexpr size_t FCL =
       (FUSED && !ST) ? 0 : sizeof(NS2<R>) * C::LP * C::FB * TS<T>::NM;
   static constexpr size_t FBUF =
-      ST      ? sizeof(C2<R>) * (C::NT / 64) * fft_buf_elems(TS<T>::NLON)
+      ST      ? sizeof(C2<R>) * C::NWF * fft_buf_elems(TS<T>::NLON)
       : FUSED ? sizeof(C2<R>) * fft_buf_elems(C::LP * C::FB * TS<T>::NLON)
               : 0;
   static constexpr size_t TWS = FUSED ? sizeof(C2<R>) * TS<T>::NLON : 0;
@@ -614,15 +618,18 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
       o[(long)(NLAT - 1 - lat) * N + 2 * m + 1] = vs.y;
     }
   } else if constexpr (ST) {
-    // ---- phase 2 (staged): wave wv transforms rows wv, wv + NW, ... one at
-    // a time in its own row buffer; the first FFT pass builds
-    // z = fc_N + i fc_S with its Hermitian extension from the compact
-    // coefficients as it reads them.  Re -> north row, Im -> south row ----
+    // ---- phase 2 (staged): wave wv < NWF transforms rows wv, wv + NWF, ...
+    // one at a time in its own row buffer (the other waves of a 16-wave
+    // block have no row buffer and wait at the block barrier); the first FFT
+    // pass builds z = fc_N + i fc_S with its Hermitian extension from the
+    // compact coefficients as it reads them.  Re -> north row, Im -> south
+    // row ----
     static_assert(N == 192 || N == 384, "the staged inverse runs the wave FFT of 192 or 384");
+    constexpr int NWF = InvCfg<T, MODE, V>::NWF;
     const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    C2<R>* wrow = fbuf + wv * NRP;
+    C2<R>* wrow = fbuf + (wv < NWF ? wv : 0) * NRP;
     typedef R R4 __attribute__((ext_vector_type(4)));
-    for (int f = wv; f < LP * FB; f += NW) {
+    for (int f = wv; wv < NWF && f < LP * FB; f += NWF) {
       const int pl = f / FB, fl = f - pl * FB;
       const int d = dst_of(fl);
       if (d < 0) continue;
