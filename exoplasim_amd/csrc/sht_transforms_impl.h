@@ -29,9 +29,28 @@ __global__ __launch_bounds__((InvCfg<T, MODE, V>::NT)) void k_inv_rt(InvArgs aq,
                                                 int gq, int gj, int gu) {
   __shared__ __align__(16) char lds[InvLds<T, MODE, true, V>::BYTES];
   constexpr int NPB = TS<T>::NPAIR / InvCfg<T, MODE, V>::LP;
-  const int b = xcd_block(blockIdx.x, gridDim.x);
   const int per = NPB * (gq + gj + gu);
-  const int mem = b / per, r = b - mem * per;
+  int mem, r;
+  if (T == 127 && aq.nmem % 8 == 0) {
+    // T127: workgroups go round-robin to the 8 XCDs, and XCD x =
+    // blockIdx.x % 8 runs members x M .. x M + M - 1 (M = nmem / 8), first
+    // all their dv2uv items, then all their scalar items: the round trip of
+    // 8 members at L40 1% faster than the member-major order below (which
+    // stays faster at T42, by about as much)
+    const int x = blockIdx.x % 8, i = blockIdx.x / 8, M = aq.nmem / 8;
+    const int nu = M * NPB * gu, ns = NPB * (gq + gj);
+    if (i < nu) {
+      mem = x * M + i / (NPB * gu);
+      r = ns + i % (NPB * gu);
+    } else {
+      mem = x * M + (i - nu) / ns;
+      r = (i - nu) % ns;
+    }
+  } else {
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    mem = b / per;
+    r = b - mem * per;
+  }
   if (r < NPB * gq)
     inv_body<T, MODE, INV_SCALAR, true, V>(aq, mem * NPB * gq + r, tq, tu, plane, tabv, twp, lds);
   else if (r < NPB * (gq + gj))
