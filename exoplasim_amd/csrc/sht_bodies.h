@@ -55,7 +55,7 @@ struct InvCfg {
   // T127 all slices of a table fit in L2 together, and the order is field
   // group major (PBT = NPAIR / LP).
   static constexpr int PBT_SC = (T == 127) ? 4 : TS<T>::NPAIR / LP;
-  static constexpr int PBT_UV = (T == 127) ? 4 : TS<T>::NPAIR / LP;
+  static constexpr int PBT_UV = (T == 127) ? 6 : TS<T>::NPAIR / LP;
 };
 
 template <int T, int MODE, int V = 0>
