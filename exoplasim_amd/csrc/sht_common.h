@@ -292,7 +292,10 @@ template <> struct FftPlan<384> { static constexpr int NP = 3; static constexpr 
 // thread)
 template <int N, int NT = 256>
 struct FftBatch {
-  static constexpr int NF = ((N == 64) ? 32 : (N == 128) ? 16 : (N == 192) ? 10 : 5) * (NT / 256);
+  // N = 192 on 512 threads: 16 rows (three batches for the 48 rows of a T63
+  // direct block), so that the block's LDS (77 KB) lets two blocks share a CU
+  static constexpr int NF =
+      ((N == 64) ? 32 : (N == 128) ? 16 : (N == 192) ? ((NT == 512) ? 8 : 10) : 5) * (NT / 256);
 };
 
 // FFT work rows in LDS hold one pad element after every 32 complex values:
