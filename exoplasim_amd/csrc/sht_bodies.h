@@ -34,8 +34,11 @@ struct InvCfg {
   // T63, 78 KB and two blocks per CU, made the 32-member inverse 4% faster
   // but the round trip no faster and the ensemble 1.5% slower.)
   static constexpr bool STAGED = (T == 127) && (MODE != MODE_F64);
+  // T63: 6 field-levels (77.6 KB of FFT rows on 512 threads, two blocks per
+  // CU, 12 of the 16 MFMA columns) beat both 8 (103 KB, one block per CU)
+  // and 4 (three blocks, half the columns)
   static constexpr int FB0 =
-      (T == 127) ? (MODE == MODE_F64 ? 2 : 8) : (MODE == MODE_F64 ? 4 : 8);
+      (T == 127) ? (MODE == MODE_F64 ? 2 : 8) : (MODE == MODE_F64 ? 4 : (T == 63 ? 6 : 8));
   static constexpr int FB = (V == 1 || V == 3) ? ((FB0 >> 1) < 2 ? 2 : (FB0 >> 1)) : FB0;
   // threads per block; the staged V = 2 block runs 16 waves in the Legendre
   // phase, of which 8 then run the FFT rows (one row buffer each)
@@ -192,7 +195,7 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
     } else {
       const int lvl = g * (FB / 2) + (fl >> 1);
       const int isv = fl & 1;
-      if (fl >= FB || lvl >= a.nlev) return R(0);
+      if (fl >= FB || (fl >> 1) >= FB / 2 || lvl >= a.nlev) return R(0);
       const R* zeta = spm + (long)(a.vor0 + lvl) * S::NRSP;
       const R* div = spm + (long)(a.div0 + lvl) * S::NRSP;
       // term 0 (qv): u <- zeta, v <- -D ; term 1 (qu): u <- -iD, v <- -i zeta
@@ -593,6 +596,9 @@ __device__ __forceinline__ void inv_body(const InvArgs& a, int blk, const void* 
       const int field = g * FB + fl;
       return field < a.nfield ? a.dst0 + field : -1;
     }
+    // a block holds FB / 2 levels (u and v rows); with FB odd its last row
+    // holds no field and must not reach the next group's first level
+    if ((fl >> 1) >= FB / 2) return -1;
     const int lvl = g * (FB / 2) + (fl >> 1);
     if (lvl >= a.nlev) return -1;
     return ((fl & 1) ? a.v0 : a.u0) + lvl;
